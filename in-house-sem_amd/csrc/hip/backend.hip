@@ -1,0 +1,1123 @@
+// Context and C-ABI of the gfx950 time-stepping backend (include/insem_hip.h).
+//
+// Device layout (chosen for MI355X, not the reference's CPU layout):
+//  * Global points of each domain are renumbered: points with no boundary,
+//    coupling, source or receiver work come first ("regular"), the rest
+//    ("special") after them.  The bulk tail kernel then runs branch-free;
+//    a small kernel handles the special points.  The numbering is internal:
+//    every field crossing the ABI is in the reference numbering.
+//  * Element-local forces f_local are [block position][component][k] with
+//    the blocks of a domain concatenated, so each force launch writes one
+//    contiguous range; the transpose CSR stores, per global point and in the
+//    reference's ascending slot order, the f_local address of each
+//    contribution.
+//  * Per-point block data (metric, coefficients, wr, hoop, r) are stored
+//    component-major; components that are bitwise constant over every
+//    element are stored once per element.
+//  * The predictor of step n+1 is fused into the tail of step n (except on
+//    the last step of an advance call), so one step is: element forces ->
+//    tail (assembly, M^-1, boundary terms, corrector, predictor) per domain,
+//    then the receivers.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../../include/insem_hip.h"
+#include "device.hpp"
+#include "launch.hpp"
+
+using namespace insem_hip;
+
+namespace {
+
+thread_local std::string g_error;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+struct HipFailure {
+  int code;
+};
+
+#define HIP_TRY(expr)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) {                                                                            \
+      set_error(e_ == hipErrorOutOfMemory ? INSEM_HIP_ENOMEM : INSEM_HIP_EDEVICE, "%s failed: %s", #expr, \
+                hipGetErrorString(e_));                                                                \
+      throw HipFailure{e_ == hipErrorOutOfMemory ? INSEM_HIP_ENOMEM : INSEM_HIP_EDEVICE};              \
+    }                                                                                                  \
+  } while (0)
+
+struct BlockRT {
+  int phys = 0;
+  bool rect = false;
+  int nb = 0;
+  int pos0 = 0;
+  ForceArgs args;
+  int32_t* d_elem = nullptr;
+};
+
+struct DomainRT {
+  int medium = 0, ncomp = 1, am = 0, npe = 1;
+  int64_t ng = 0, n_reg = 0, n_sp = 0, ne = 0;
+  std::vector<int32_t> perm_new, perm_old;
+  std::vector<uint8_t> h_dir_mask;  // per old point: 1 Dirichlet, 2 axis (host copy for set_state)
+  int axis_mask = 0;
+  DomDev dd;
+  SpecialDev sp;
+  std::vector<BlockRT> blocks;
+  float* tmp = nullptr;  // [ncomp][ng] scratch (seams, power iteration)
+  float* fl_tmp = nullptr;
+  int32_t* d_dir = nullptr;
+  int n_dir = 0;
+  int32_t* d_axis = nullptr;
+  int n_axis = 0;
+  // power-iteration coupling (reference order)
+  int32_t *cpl_pts = nullptr, *cpl_other = nullptr;
+  float *cpl_cx = nullptr, *cpl_cz = nullptr;
+  int n_cpl = 0;
+  int cpl_role = 0;
+};
+
+struct TraceCol {
+  int receiver, quantity, raw_ncomp, col, m1_elastic;
+};
+
+}  // namespace
+
+struct insem_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  int64_t footprint = 0;
+  int ngll = 0;
+  float *D = nullptr, *Dj = nullptr, *w2 = nullptr;
+  std::vector<DomainRT> doms;
+  int solid = -1, fluid = -1;
+  int n_sources = 0;
+  // receivers
+  ReceiverDev rec;
+  std::vector<TraceCol> cols;
+  int has_pv = 0;
+  int n_traces = 0;
+  // time
+  double dt = 0.0;
+  int n_steps = 0, step = 0, initialized = 0, have_time = 0;
+  float* amps = nullptr;
+  int record_every = 1, energy_every = 0;
+  int n_samples = 0, max_samples = 0;
+  float* traces = nullptr;
+  // energy
+  double* partial = nullptr;
+  int partial_rows = 0;
+  double* energy_rows = nullptr;  // [row][2]: sum u.f, kinetic
+  std::vector<int> energy_steps;
+  int max_energy_rows = 0;
+  double* scratch = nullptr;  // small reduction outputs
+  // timing
+  int timing = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_force, ev_update;
+  std::vector<int> ev_force_tag;           // domain * 64 + block of each force event pair
+  std::map<int, std::pair<double, int>> block_ms;
+  int cur_tag = 0;
+  std::vector<hipEvent_t> ev_pool;
+  double force_ms = 0.0, update_ms = 0.0;
+  int force_n = 0, update_n = 0;
+
+  template <class T>
+  T* alloc(size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, n * sizeof(T)));
+    allocs.push_back(p);
+    footprint += static_cast<int64_t>(n * sizeof(T));
+    return static_cast<T*>(p);
+  }
+  template <class T>
+  T* upload(const T* src, size_t n) {
+    T* d = alloc<T>(n);
+    if (n) HIP_TRY(hipMemcpy(d, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+  }
+  template <class T>
+  T* upload(const std::vector<T>& v) {
+    return upload(v.data(), v.size());
+  }
+  hipEvent_t event() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    return e;
+  }
+};
+
+namespace {
+
+bool bits_equal(float a, float b) {
+  uint32_t x, y;
+  std::memcpy(&x, &a, 4);
+  std::memcpy(&y, &b, 4);
+  return x == y;
+}
+
+// Uploads one per-point quantity get(b, k) of a block, per element when it is
+// bitwise constant over every element.
+template <class Get>
+PF upload_pf(insem_hip_ctx* c, int nb, int npe, Get get) {
+  bool per_elem = true;
+  for (int b = 0; b < nb && per_elem; ++b) {
+    const float v0 = get(b, 0);
+    for (int k = 1; k < npe; ++k)
+      if (!bits_equal(get(b, k), v0)) {
+        per_elem = false;
+        break;
+      }
+  }
+  PF pf;
+  if (per_elem) {
+    std::vector<float> h(static_cast<size_t>(nb));
+    for (int b = 0; b < nb; ++b) h[b] = get(b, 0);
+    pf.p = c->upload(h);
+    pf.sb = 1;
+    pf.sk = 0;
+  } else {
+    std::vector<float> h(static_cast<size_t>(nb) * npe);
+    for (int b = 0; b < nb; ++b)
+      for (int k = 0; k < npe; ++k) h[static_cast<size_t>(b) * npe + k] = get(b, k);
+    pf.p = c->upload(h);
+    pf.sb = npe;
+    pf.sk = 1;
+  }
+  return pf;
+}
+
+int phys_of(int medium, int kind, int am) {
+  const bool el = medium == INSEM_MEDIUM_ELASTIC;
+  if (kind == INSEM_KIND_AXISYM) {
+    if (am == 1) return el ? kElasticAxi1 : kAcousticAxi1;
+    return el ? kElasticAxi0 : kAcousticAxi0;
+  }
+  return el ? kElasticCart : kAcousticCart;
+}
+
+void free_all(insem_hip_ctx* c) {
+  for (void* p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  for (auto& e : c->ev_force) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for (auto& e : c->ev_update) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+int validate_setup(const insem_hip_setup* s) {
+  if (!s) return set_error(INSEM_HIP_EINVAL, "null setup");
+  if (s->ngll < 2 || s->ngll > 9) return set_error(INSEM_HIP_EINVAL, "no kernel for %d GLL points (supported: 2..9)", s->ngll);
+  if (s->n_domains < 0 || s->n_domains > 2) return set_error(INSEM_HIP_EINVAL, "a model has one or two domains");
+  if (s->record_every < 1) return set_error(INSEM_HIP_EINVAL, "record_every must be >= 1");
+  if (s->energy_every < 0) return set_error(INSEM_HIP_EINVAL, "energy_every must be >= 0");
+  int seen[2] = {0, 0};
+  for (int di = 0; di < s->n_domains; ++di) {
+    const insem_hip_domain& d = s->domains[di];
+    if (d.medium != INSEM_MEDIUM_ACOUSTIC && d.medium != INSEM_MEDIUM_ELASTIC)
+      return set_error(INSEM_HIP_EINVAL, "domain %d has an unknown medium", di);
+    if (seen[d.medium]++) return set_error(INSEM_HIP_EINVAL, "two domains of the same medium");
+    const int want = d.medium == INSEM_MEDIUM_ELASTIC ? (d.azimuthal_order == 1 ? 3 : 2) : 1;
+    if (d.ncomp != want) return set_error(INSEM_HIP_EINVAL, "domain %d has %d components, expected %d", di, d.ncomp, want);
+    int64_t covered = 0;
+    for (int b = 0; b < d.n_blocks; ++b) {
+      const insem_hip_block& B = d.blocks[b];
+      if (!(B.azimuthal_order == 0 || (B.azimuthal_order == 1 && B.kind == INSEM_KIND_AXISYM)))
+        return set_error(INSEM_HIP_EINVAL, "no kernel for azimuthal order %d on kind %d elements", B.azimuthal_order,
+                         B.kind);
+      if (B.kind < 0 || B.kind > 3) return set_error(INSEM_HIP_EINVAL, "unknown geometry kind %d", B.kind);
+      if (B.kind == INSEM_KIND_AXISYM && (!B.wr || !B.hoop || !B.r || !B.axial))
+        return set_error(INSEM_HIP_EINVAL, "axisymmetric block without wr/hoop/r/axial");
+      for (int e = 0; e < B.n_elements; ++e)
+        if (B.elements[e] < 0 || B.elements[e] >= d.n_elements)
+          return set_error(INSEM_HIP_EINVAL, "block element index out of range");
+      covered += B.n_elements;
+    }
+    if (covered != d.n_elements) return set_error(INSEM_HIP_EINVAL, "blocks of domain %d do not cover its elements", di);
+  }
+  for (int i = 0; i < s->n_sources; ++i)
+    if (s->sources[i].domain < 0 || s->sources[i].domain >= s->n_domains)
+      return set_error(INSEM_HIP_EINVAL, "source %d has an invalid domain", i);
+  for (int i = 0; i < s->n_receivers; ++i)
+    if (s->receivers[i].domain < 0 || s->receivers[i].domain >= s->n_domains)
+      return set_error(INSEM_HIP_EINVAL, "receiver %d has an invalid domain", i);
+  return INSEM_HIP_OK;
+}
+
+void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
+  const insem_hip_domain& d = s->domains[di];
+  DomainRT& R = c->doms[di];
+  const int npe = s->ngll * s->ngll;
+  const int C = d.ncomp;
+  R.medium = d.medium;
+  R.ncomp = C;
+  R.am = d.azimuthal_order;
+  R.npe = npe;
+  R.ng = d.n_global;
+  R.ne = d.n_elements;
+
+  // Special points of this domain.
+  std::vector<uint8_t> special(static_cast<size_t>(d.n_global), 0);
+  auto mark = [&](const int32_t* p, int n) {
+    for (int i = 0; i < n; ++i) special[p[i]] = 1;
+  };
+  mark(d.absorbing_points, d.n_absorbing);
+  mark(d.dirichlet_points, d.n_dirichlet);
+  mark(d.axis_points, d.n_axis);
+  for (int i = 0; i < s->n_sources; ++i)
+    if (s->sources[i].domain == di) mark(s->sources[i].points, s->sources[i].n_points);
+  for (int i = 0; i < s->n_receivers; ++i)
+    if (s->receivers[i].domain == di) mark(s->receivers[i].points, s->receivers[i].n_points);
+  const bool is_solid = di == c->solid, is_fluid = di == c->fluid;
+  if (s->coupling.n_points > 0) {
+    if (is_solid) mark(s->coupling.solid_points, s->coupling.n_points);
+    if (is_fluid) mark(s->coupling.fluid_points, s->coupling.n_points);
+  }
+  R.perm_new.assign(d.n_global, -1);
+  R.perm_old.clear();
+  R.perm_old.reserve(d.n_global);
+  for (int pass = 0; pass < 2; ++pass)
+    for (int32_t g = 0; g < d.n_global; ++g)
+      if (special[g] == pass) {
+        R.perm_new[g] = static_cast<int32_t>(R.perm_old.size());
+        R.perm_old.push_back(g);
+      }
+  R.n_sp = 0;
+  for (uint8_t v : special) R.n_sp += v;
+  R.n_reg = R.ng - R.n_sp;
+
+  // Element positions: blocks concatenated.
+  std::vector<int32_t> pos(d.n_elements, -1);
+  {
+    int32_t next = 0;
+    for (int b = 0; b < d.n_blocks; ++b)
+      for (int e = 0; e < d.blocks[b].n_elements; ++e) pos[d.blocks[b].elements[e]] = next++;
+  }
+
+  // Transpose CSR in the new numbering, slot order kept.
+  std::vector<int32_t> off(d.n_global + 1, 0), addr;
+  std::vector<float> tw;
+  addr.reserve(d.t_offsets[d.n_global]);
+  for (int64_t gn = 0; gn < R.ng; ++gn) {
+    const int32_t go = R.perm_old[gn];
+    for (int32_t t = d.t_offsets[go]; t < d.t_offsets[go + 1]; ++t) {
+      const int32_t slot = d.t_slots[t];
+      const int32_t e = slot / npe, k = slot - e * npe;
+      addr.push_back((pos[e] * C) * npe + k);
+      if (d.t_weights) tw.push_back(static_cast<float>(d.t_weights[t]));
+    }
+    off[gn + 1] = static_cast<int32_t>(addr.size());
+  }
+  // Fields, mass, CSR.
+  R.dd.ncomp = C;
+  R.dd.npe = npe;
+  R.dd.ng = R.ng;
+  R.dd.n_reg = R.n_reg;
+  const size_t len = static_cast<size_t>(C) * R.ng;
+  R.dd.u = c->alloc<float>(len);
+  R.dd.v = c->alloc<float>(len);
+  R.dd.a = c->alloc<float>(len);
+  R.dd.f = c->alloc<float>(len);
+  R.tmp = c->alloc<float>(len);
+  HIP_TRY(hipMemset(R.dd.u, 0, len * sizeof(float)));
+  HIP_TRY(hipMemset(R.dd.v, 0, len * sizeof(float)));
+  HIP_TRY(hipMemset(R.dd.a, 0, len * sizeof(float)));
+  HIP_TRY(hipMemset(R.dd.f, 0, len * sizeof(float)));
+  std::vector<float> im(R.ng), ms(R.ng);
+  for (int64_t gn = 0; gn < R.ng; ++gn) {
+    im[gn] = d.inv_mass[R.perm_old[gn]];
+    ms[gn] = d.mass[R.perm_old[gn]];
+  }
+  R.dd.im = c->upload(im);
+  R.dd.mass = c->upload(ms);
+  R.dd.t_off = c->upload(off);
+  R.dd.t_addr = c->upload(addr);
+  R.dd.t_w = d.t_weights ? c->upload(tw) : nullptr;
+  float* fl = c->alloc<float>(static_cast<size_t>(d.n_elements) * C * npe);
+  HIP_TRY(hipMemset(fl, 0, static_cast<size_t>(d.n_elements) * C * npe * sizeof(float)));
+  R.dd.fl = fl;
+  R.fl_tmp = c->alloc<float>(static_cast<size_t>(d.n_elements) * C * npe);
+
+  // Constraint rows (new numbering, float weights as the reference's gather uses them).
+  const int32_t* row_off = nullptr;
+  const int32_t* row_dofs = nullptr;
+  const float* row_w = nullptr;
+  if (d.n_rows > 0) {
+    const int32_t nnz = d.row_offsets[d.n_rows];
+    std::vector<int32_t> rd(nnz);
+    std::vector<float> rw(nnz);
+    for (int32_t q = 0; q < nnz; ++q) {
+      rd[q] = R.perm_new[d.row_dofs[q]];
+      rw[q] = static_cast<float>(d.row_weights[q]);
+    }
+    row_off = c->upload(d.row_offsets, static_cast<size_t>(d.n_rows) + 1);
+    row_dofs = c->upload(rd);
+    row_w = c->upload(rw);
+  }
+
+  // Blocks.
+  for (int bi = 0; bi < d.n_blocks; ++bi) {
+    const insem_hip_block& B = d.blocks[bi];
+    BlockRT br;
+    br.nb = B.n_elements;
+    br.pos0 = B.n_elements ? pos[B.elements[0]] : 0;
+    br.phys = phys_of(d.medium, B.kind, B.azimuthal_order);
+    br.rect = B.kind == INSEM_KIND_RECT;
+    const int nb = B.n_elements;
+    std::vector<int32_t> l2g(static_cast<size_t>(nb) * npe);
+    for (int b = 0; b < nb; ++b)
+      for (int k = 0; k < npe; ++k) {
+        const int32_t m = d.l2g[static_cast<size_t>(B.elements[b]) * npe + k];
+        l2g[static_cast<size_t>(b) * npe + k] = m >= 0 ? R.perm_new[m] : m;
+      }
+    ForceArgs& A = br.args;
+    A.nb = nb;
+    A.l2g = c->upload(l2g);
+    A.row_off = row_off;
+    A.row_dofs = row_dofs;
+    A.row_w = row_w;
+    A.u = R.dd.u;
+    A.ng = R.ng;
+    br.d_elem = c->upload(B.elements, static_cast<size_t>(nb));
+    A.elem = br.d_elem;
+    A.fl = fl;
+    A.out_pos0 = br.pos0;
+    A.D = c->D;
+    A.Dj = c->Dj;
+    A.w2 = c->w2;
+    const float* M = B.metric;
+    if (B.kind == INSEM_KIND_RECT) {
+      A.m[0] = upload_pf(c, nb, 1, [&](int b, int) { return M[3 * b]; });
+      A.m[3] = upload_pf(c, nb, 1, [&](int b, int) { return M[3 * b + 1]; });
+      A.m[4] = upload_pf(c, nb, 1, [&](int b, int) { return M[3 * b + 2]; });
+      for (int q : {0, 3, 4}) A.m[q].sk = 0;
+    } else if (B.kind == INSEM_KIND_AFFINE) {
+      for (int q = 0; q < 5; ++q) {
+        A.m[q] = upload_pf(c, nb, 1, [&](int b, int) { return M[5 * b + q]; });
+        A.m[q].sk = 0;
+      }
+    } else {
+      const int ncm = B.kind == INSEM_KIND_AXISYM ? 4 : 5;  // axisymmetric kernels use wr, not jac
+      for (int q = 0; q < ncm; ++q)
+        A.m[q] = upload_pf(c, nb, npe, [&](int b, int k) { return M[(static_cast<size_t>(b) * npe + k) * 5 + q]; });
+    }
+    A.c0 = upload_pf(c, nb, npe, [&](int b, int k) { return B.coef0[static_cast<size_t>(b) * npe + k]; });
+    if (d.medium == INSEM_MEDIUM_ELASTIC)
+      A.c1 = upload_pf(c, nb, npe, [&](int b, int k) { return B.coef1[static_cast<size_t>(b) * npe + k]; });
+    if (B.kind == INSEM_KIND_AXISYM) {
+      A.wr = upload_pf(c, nb, npe, [&](int b, int k) { return B.wr[static_cast<size_t>(b) * npe + k]; });
+      const bool need_hoop = !(d.medium == INSEM_MEDIUM_ACOUSTIC && B.azimuthal_order == 0);
+      if (need_hoop) A.hoop = upload_pf(c, nb, npe, [&](int b, int k) { return B.hoop[static_cast<size_t>(b) * npe + k]; });
+      A.r = upload_pf(c, nb, npe, [&](int b, int k) { return B.r[static_cast<size_t>(b) * npe + k]; });
+      A.axial = c->upload(B.axial, static_cast<size_t>(nb));
+    }
+    R.blocks.push_back(br);
+  }
+
+  // Dirichlet / axis.
+  R.h_dir_mask.assign(d.n_global, 0);
+  for (int i = 0; i < d.n_dirichlet; ++i) R.h_dir_mask[d.dirichlet_points[i]] |= 1;
+  for (int i = 0; i < d.n_axis; ++i) R.h_dir_mask[d.axis_points[i]] |= 2;
+  R.axis_mask = 0;
+  for (int i = 0; i < d.n_axis_components; ++i) R.axis_mask |= 1 << d.axis_components[i];
+  {
+    std::vector<int32_t> p(d.n_dirichlet), q(d.n_axis);
+    for (int i = 0; i < d.n_dirichlet; ++i) p[i] = R.perm_new[d.dirichlet_points[i]];
+    for (int i = 0; i < d.n_axis; ++i) q[i] = R.perm_new[d.axis_points[i]];
+    R.d_dir = c->upload(p);
+    R.n_dir = d.n_dirichlet;
+    R.d_axis = c->upload(q);
+    R.n_axis = d.n_axis;
+  }
+
+  // Special-point tables.
+  const int64_t nsp = R.n_sp;
+  auto sidx = [&](int32_t g_old) { return static_cast<int64_t>(R.perm_new[g_old]) - R.n_reg; };
+  std::vector<int32_t> abs_idx(nsp, -1), cpl_idx(nsp, -1);
+  std::vector<uint8_t> flags(nsp, 0);
+  for (int i = 0; i < d.n_absorbing; ++i) abs_idx[sidx(d.absorbing_points[i])] = i;
+  for (int i = 0; i < d.n_dirichlet; ++i) flags[sidx(d.dirichlet_points[i])] |= 1;
+  for (int i = 0; i < d.n_axis; ++i) flags[sidx(d.axis_points[i])] |= 2;
+  std::vector<std::vector<std::pair<int32_t, std::vector<float>>>> src(nsp);
+  for (int si = 0; si < s->n_sources; ++si) {
+    const insem_hip_source& S = s->sources[si];
+    if (S.domain != di) continue;
+    for (int i = 0; i < S.n_points; ++i) {
+      std::vector<float> cf(C);
+      for (int cc = 0; cc < C; ++cc) cf[cc] = S.coef[static_cast<size_t>(cc) * S.n_points + i];
+      src[sidx(S.points[i])].push_back({si, cf});
+    }
+  }
+  std::vector<int32_t> soff(nsp + 1, 0), sid;
+  std::vector<float> scoef;
+  for (int64_t i = 0; i < nsp; ++i) {
+    for (auto& e : src[i]) {
+      sid.push_back(e.first);
+      scoef.insert(scoef.end(), e.second.begin(), e.second.end());
+    }
+    soff[i + 1] = static_cast<int32_t>(sid.size());
+  }
+  SpecialDev& sp = R.sp;
+  sp.n_sp = nsp;
+  sp.abs_idx = c->upload(abs_idx);
+  sp.abs_coef = c->upload(d.absorbing_coef, static_cast<size_t>(d.n_absorbing) * C * C);
+  sp.flags = c->upload(flags);
+  sp.axis_mask = R.axis_mask;
+  sp.src_off = c->upload(soff);
+  sp.src_id = c->upload(sid);
+  sp.src_coef = c->upload(scoef);
+  sp.pm = R.am == 1;
+  sp.su = c->alloc<float>(static_cast<size_t>(C) * nsp);
+  sp.sv = c->alloc<float>(static_cast<size_t>(C) * nsp);
+  sp.sa = c->alloc<float>(static_cast<size_t>(C) * nsp);
+  sp.cpl_idx = nullptr;
+  sp.cpl_role = 0;
+  R.n_cpl = 0;
+  R.cpl_role = 0;
+  (void)is_fluid;
+  sp.cpl_idx = c->upload(cpl_idx);  // filled after both domains exist
+}
+
+// Coupling tables need the numbering of both domains.
+void build_coupling(insem_hip_ctx* c, const insem_hip_setup* s) {
+  const insem_hip_coupling& cp = s->coupling;
+  if (cp.n_points == 0 || c->solid < 0 || c->fluid < 0) return;
+  for (int role = 1; role <= 2; ++role) {
+    DomainRT& R = c->doms[role == 1 ? c->solid : c->fluid];
+    DomainRT& O = c->doms[role == 1 ? c->fluid : c->solid];
+    const int32_t* mine = role == 1 ? cp.solid_points : cp.fluid_points;
+    const int32_t* theirs = role == 1 ? cp.fluid_points : cp.solid_points;
+    std::vector<int32_t> idx(R.n_sp, -1), pts(cp.n_points), other(cp.n_points);
+    for (int j = 0; j < cp.n_points; ++j) {
+      const int64_t si = static_cast<int64_t>(R.perm_new[mine[j]]) - R.n_reg;
+      if (idx[si] >= 0) {
+        set_error(INSEM_HIP_EINVAL, "coupling maps two interface entries to one %s point",
+                  role == 1 ? "solid" : "fluid");
+        throw HipFailure{INSEM_HIP_EINVAL};
+      }
+      idx[si] = j;
+      pts[j] = R.perm_new[mine[j]];
+      other[j] = O.perm_new[theirs[j]];
+    }
+    R.sp.cpl_idx = c->upload(idx);
+    R.sp.cpl_other = c->upload(other);
+    R.sp.cpl_cx = c->upload(cp.cx, cp.n_points);
+    R.sp.cpl_cz = c->upload(cp.cz, cp.n_points);
+    R.sp.cpl_role = role;
+    R.sp.other = role == 1 ? O.dd.u : O.dd.a;
+    R.sp.other_ng = O.ng;
+    R.cpl_pts = c->upload(pts);
+    R.cpl_other = const_cast<int32_t*>(R.sp.cpl_other);
+    R.cpl_cx = const_cast<float*>(R.sp.cpl_cx);
+    R.cpl_cz = const_cast<float*>(R.sp.cpl_cz);
+    R.n_cpl = cp.n_points;
+    R.cpl_role = role;
+  }
+}
+
+void build_receivers(insem_hip_ctx* c, const insem_hip_setup* s) {
+  std::vector<int32_t> dom, kind, nc, col, off{0}, pts;
+  std::vector<float> val, ddx, ddz, rho;
+  int ncols = 0;
+  for (int ri = 0; ri < s->n_receivers; ++ri) {
+    const insem_hip_receiver& r = s->receivers[ri];
+    const DomainRT& R = c->doms[r.domain];
+    for (int q = 0; q < r.n_quantities; ++q) {
+      const int k = r.quantities[q];
+      const int raw = k == INSEM_Q_PARTICLE_VELOCITY ? 2 : R.ncomp;
+      dom.push_back(r.domain);
+      kind.push_back(k);
+      nc.push_back(raw);
+      col.push_back(ncols);
+      c->cols.push_back({ri, q, raw, ncols, R.medium == INSEM_MEDIUM_ELASTIC && R.am == 1 ? 1 : 0});
+      ncols += raw;
+      if (k == INSEM_Q_PARTICLE_VELOCITY) c->has_pv = 1;
+      for (int i = 0; i < r.n_points; ++i) {
+        pts.push_back(static_cast<int32_t>(R.perm_new[r.points[i]] - R.n_reg));
+        val.push_back(r.value[i]);
+        ddx.push_back(r.d_dx[i]);
+        ddz.push_back(r.d_dz[i]);
+      }
+      off.push_back(static_cast<int32_t>(pts.size()));
+      rho.push_back(static_cast<float>(r.rho));
+    }
+  }
+  ReceiverDev& rd = c->rec;
+  rd.n_tasks = static_cast<int>(dom.size());
+  c->n_traces = rd.n_tasks;
+  rd.domain = c->upload(dom);
+  rd.kind = c->upload(kind);
+  rd.ncomp = c->upload(nc);
+  rd.col = c->upload(col);
+  rd.pt_off = c->upload(off);
+  rd.pts = c->upload(pts);
+  rd.value = c->upload(val);
+  rd.d_dx = c->upload(ddx);
+  rd.d_dz = c->upload(ddz);
+  rd.rho = c->upload(rho);
+  rd.pv = c->alloc<float>(6 * static_cast<size_t>(std::max(1, rd.n_tasks)));
+  HIP_TRY(hipMemset(rd.pv, 0, 6 * static_cast<size_t>(std::max(1, rd.n_tasks)) * sizeof(float)));
+  rd.n_cols = ncols;
+  for (size_t di = 0; di < c->doms.size(); ++di) {
+    rd.su[di] = c->doms[di].sp.su;
+    rd.sv[di] = c->doms[di].sp.sv;
+    rd.sa[di] = c->doms[di].sp.sa;
+    rd.n_sp[di] = c->doms[di].n_sp;
+  }
+}
+
+// ------------------------------------------------------------ step pieces
+
+void timed_begin(insem_hip_ctx* c, hipEvent_t* e) {
+  if (!c->timing) return;
+  *e = c->event();
+  HIP_TRY(hipEventRecord(*e, c->stream));
+}
+
+void timed_end(insem_hip_ctx* c, hipEvent_t b, bool force) {
+  if (!c->timing) return;
+  hipEvent_t e = c->event();
+  HIP_TRY(hipEventRecord(e, c->stream));
+  (force ? c->ev_force : c->ev_update).push_back({b, e});
+  if (force) c->ev_force_tag.push_back(c->cur_tag);
+}
+
+void run_forces(insem_hip_ctx* c, DomainRT& R, const float* u) {
+  const int di = static_cast<int>(&R - c->doms.data());
+  for (size_t bi = 0; bi < R.blocks.size(); ++bi) {
+    BlockRT& br = R.blocks[bi];
+    c->cur_tag = di * 64 + static_cast<int>(bi);
+    ForceArgs a = br.args;
+    a.u = u;
+    hipEvent_t e0 = nullptr;
+    timed_begin(c, &e0);
+    HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, true, a, c->stream));
+    timed_end(c, e0, true);
+  }
+}
+
+StepCoef step_coef(const insem_hip_ctx* c, bool implicit) {
+  StepCoef sc;
+  sc.rdt = static_cast<float>(c->dt);
+  sc.hdt = static_cast<float>(0.5 * c->dt);
+  sc.hdt2 = static_cast<float>(0.5 * c->dt * c->dt);
+  sc.h_damp = implicit ? static_cast<float>(0.5 * c->dt) : 0.0f;
+  sc.cc = c->dt * c->dt / 8.0;
+  return sc;
+}
+
+// One compute_acceleration + corrector (+ predictor) over both domains.
+void step_body(insem_hip_ctx* c, int step, bool implicit, int corr, int pred, int energy) {
+  const StepCoef sc = step_coef(c, implicit);
+  int row = 0;
+  for (int medium : {INSEM_MEDIUM_ELASTIC, INSEM_MEDIUM_ACOUSTIC}) {
+    const int di = medium == INSEM_MEDIUM_ELASTIC ? c->solid : c->fluid;
+    if (di < 0) continue;
+    DomainRT& R = c->doms[di];
+    run_forces(c, R, R.dd.u);
+    int nb = 0;
+    hipEvent_t e0 = nullptr;
+    timed_begin(c, &e0);
+    HIP_TRY(launch_tail_regular(R.dd, sc, corr, pred, energy, c->partial + 2 * row, &nb, c->stream));
+    timed_end(c, e0, false);
+    row += energy ? nb : 0;
+    HIP_TRY(launch_tail_special(R.dd, R.sp, sc, step, corr, pred, energy, c->partial + 2 * row, &nb, c->stream));
+    row += energy ? nb : 0;
+  }
+  if (energy) {
+    const int r = static_cast<int>(c->energy_steps.size());
+    HIP_TRY(launch_reduce(c->partial, row, 2, c->energy_rows + 2 * r, 0, c->stream));
+    c->energy_steps.push_back(step);
+  }
+}
+
+void record(insem_hip_ctx* c, int step, int pv_mode) {
+  const bool sample = step % c->record_every == 0;
+  if (!sample && !c->has_pv) return;
+  const int idx = sample ? step / c->record_every : -1;
+  HIP_TRY(launch_receivers(c->rec, idx, pv_mode, static_cast<float>(0.5 * c->dt), c->stream));
+  if (sample) c->n_samples = idx + 1;
+}
+
+void initialize(insem_hip_ctx* c) {
+  const int energy = c->energy_every > 0;
+  step_body(c, 0, false, 0, 0, energy);
+  record(c, 0, 0);
+  c->initialized = 1;
+}
+
+void predictor_all(insem_hip_ctx* c) {
+  const StepCoef sc = step_coef(c, true);
+  for (DomainRT& R : c->doms) HIP_TRY(launch_predictor(R.dd, sc, c->stream));
+}
+
+void require_time(insem_hip_ctx* c) {
+  if (!c->have_time) {
+    set_error(INSEM_HIP_ESTATE, "insem_hip_set_time has not been called");
+    throw HipFailure{INSEM_HIP_ESTATE};
+  }
+}
+
+template <class F>
+int guarded(insem_hip_ctx* c, F&& f) {
+  if (!c) return set_error(INSEM_HIP_EINVAL, "null context");
+  try {
+    if (c->stream) HIP_TRY(hipSetDevice(c->device));
+    f();
+    return INSEM_HIP_OK;
+  } catch (const HipFailure& e) {
+    return e.code;
+  } catch (const std::exception& e) {
+    return set_error(INSEM_HIP_EDEVICE, "%s", e.what());
+  }
+}
+
+void permute_in(const DomainRT& R, const float* src, std::vector<float>& out) {
+  out.resize(static_cast<size_t>(R.ncomp) * R.ng);
+  for (int cc = 0; cc < R.ncomp; ++cc)
+    for (int64_t gn = 0; gn < R.ng; ++gn) out[cc * R.ng + gn] = src[cc * R.ng + R.perm_old[gn]];
+}
+
+void permute_out(const DomainRT& R, const std::vector<float>& src, float* out) {
+  for (int cc = 0; cc < R.ncomp; ++cc)
+    for (int64_t gn = 0; gn < R.ng; ++gn) out[cc * R.ng + R.perm_old[gn]] = src[cc * R.ng + gn];
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+
+extern "C" {
+
+const char* insem_hip_last_error(void) { return g_error.c_str(); }
+
+int insem_hip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) {
+  if (!out) return set_error(INSEM_HIP_EINVAL, "null output pointer");
+  *out = nullptr;
+  const int v = validate_setup(s);
+  if (v != INSEM_HIP_OK) return v;
+  if (insem_hip_device_count() <= 0) return set_error(INSEM_HIP_EDEVICE, "no HIP device is visible");
+  auto* c = new insem_hip_ctx();
+  try {
+    HIP_TRY(hipGetDevice(&c->device));
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->ngll = s->ngll;
+    const size_t npe = static_cast<size_t>(s->ngll) * s->ngll;
+    c->D = c->upload(s->D, npe);
+    c->Dj = c->upload(s->Dj, npe);
+    c->w2 = c->upload(s->w2, npe);
+    c->doms.resize(s->n_domains);
+    for (int di = 0; di < s->n_domains; ++di) {
+      if (s->domains[di].medium == INSEM_MEDIUM_ELASTIC) c->solid = di;
+      if (s->domains[di].medium == INSEM_MEDIUM_ACOUSTIC) c->fluid = di;
+    }
+    for (int di = 0; di < s->n_domains; ++di) build_domain(c, s, di);
+    build_coupling(c, s);
+    build_receivers(c, s);
+    c->n_sources = s->n_sources;
+    c->record_every = s->record_every;
+    c->energy_every = s->energy_every;
+    int rows = 0;
+    for (const DomainRT& R : c->doms) rows += tail_blocks(std::max(R.n_reg, R.ng)) + tail_blocks(R.n_sp) + 1;
+    c->partial_rows = rows;
+    int dot_rows = 0;
+    for (const DomainRT& R : c->doms) dot_rows = std::max(dot_rows, tail_blocks(R.ncomp * R.ng));
+    c->partial = c->alloc<double>(3 * static_cast<size_t>(std::max(rows, dot_rows)) + 16);
+    c->scratch = c->alloc<double>(64);
+    HIP_TRY(hipDeviceSynchronize());
+  } catch (const HipFailure& e) {
+    free_all(c);
+    delete c;
+    return e.code;
+  } catch (const std::exception& e) {
+    free_all(c);
+    delete c;
+    return set_error(INSEM_HIP_EDEVICE, "%s", e.what());
+  }
+  *out = c;
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_destroy(insem_hip_ctx* c) {
+  if (!c) return INSEM_HIP_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_set_time(insem_hip_ctx* c, double dt, int32_t n_steps, const float* amplitudes) {
+  if (c && c->initialized) return set_error(INSEM_HIP_ESTATE, "the time step must be set before the first step");
+  if (!(dt > 0.0) || n_steps < 0) return set_error(INSEM_HIP_EINVAL, "invalid time step or step count");
+  if (c && c->n_sources > 0 && !amplitudes) return set_error(INSEM_HIP_EINVAL, "source amplitudes are required");
+  return guarded(c, [&] {
+    c->dt = dt;
+    c->n_steps = n_steps;
+    const size_t na = static_cast<size_t>(c->n_sources) * (static_cast<size_t>(n_steps) + 1);
+    c->amps = c->upload(amplitudes ? amplitudes : nullptr, amplitudes ? na : 0);
+    for (DomainRT& R : c->doms) {
+      R.sp.amps = c->amps;
+      R.sp.amp_stride = n_steps + 1;
+    }
+    c->max_samples = n_steps / c->record_every + 1;
+    c->traces = c->alloc<float>(static_cast<size_t>(c->max_samples) * std::max(1, c->rec.n_cols));
+    HIP_TRY(hipMemset(c->traces, 0, static_cast<size_t>(c->max_samples) * std::max(1, c->rec.n_cols) * sizeof(float)));
+    c->rec.traces = c->traces;
+    c->max_energy_rows = c->energy_every > 0 ? n_steps / c->energy_every + 1 : 0;
+    c->energy_rows = c->alloc<double>(2 * static_cast<size_t>(std::max(1, c->max_energy_rows)));
+    c->have_time = 1;
+  });
+}
+
+int insem_hip_set_state(insem_hip_ctx* c, int32_t di, const float* u, const float* v) {
+  if (!c || di < 0 || di >= static_cast<int>(c->doms.size())) return set_error(INSEM_HIP_EINVAL, "invalid domain");
+  if (c->initialized) return set_error(INSEM_HIP_ESTATE, "initial conditions must be set before the first time step");
+  if (!u || !v) return set_error(INSEM_HIP_EINVAL, "null state array");
+  return guarded(c, [&] {
+    DomainRT& R = c->doms[di];
+    std::vector<float> hu, hv;
+    permute_in(R, u, hu);
+    permute_in(R, v, hv);
+    // Dirichlet and axis points are forced to zero (solver.cpp:776-777).
+    for (int64_t gn = 0; gn < R.ng; ++gn) {
+      const uint8_t m = R.h_dir_mask[R.perm_old[gn]];
+      for (int cc = 0; cc < R.ncomp; ++cc)
+        if ((m & 1) || ((m & 2) && ((R.axis_mask >> cc) & 1))) hu[cc * R.ng + gn] = hv[cc * R.ng + gn] = 0.0f;
+    }
+    HIP_TRY(hipMemcpy(R.dd.u, hu.data(), hu.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(R.dd.v, hv.data(), hv.size() * sizeof(float), hipMemcpyHostToDevice));
+  });
+}
+
+int insem_hip_get_field(insem_hip_ctx* c, int32_t di, int32_t field, float* out) {
+  if (!c || di < 0 || di >= static_cast<int>(c->doms.size())) return set_error(INSEM_HIP_EINVAL, "invalid domain");
+  if (!out || field < 0 || field > 3) return set_error(INSEM_HIP_EINVAL, "invalid field or output");
+  return guarded(c, [&] {
+    DomainRT& R = c->doms[di];
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const float* src = field == INSEM_FIELD_U ? R.dd.u : field == INSEM_FIELD_V ? R.dd.v : field == INSEM_FIELD_A ? R.dd.a : R.dd.f;
+    std::vector<float> h(static_cast<size_t>(R.ncomp) * R.ng);
+    HIP_TRY(hipMemcpy(h.data(), src, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    permute_out(R, h, out);
+  });
+}
+
+int insem_hip_advance(insem_hip_ctx* c, int32_t n) {
+  return guarded(c, [&] {
+    require_time(c);
+    if (!c->initialized) initialize(c);
+    const int m = std::max(0, std::min(n, c->n_steps - c->step));
+    for (int s = 0; s < m; ++s) {
+      if (s == 0) predictor_all(c);
+      ++c->step;
+      const int energy = c->energy_every > 0 && c->step % c->energy_every == 0;
+      step_body(c, c->step, true, 1, s < m - 1 ? 1 : 0, energy);
+      record(c, c->step, 1);
+    }
+  });
+}
+
+int insem_hip_step(insem_hip_ctx* c, int32_t* step) {
+  if (!c || !step) return set_error(INSEM_HIP_EINVAL, "null argument");
+  *step = c->step;
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_n_samples(insem_hip_ctx* c, int32_t* n) {
+  if (!c || !n) return set_error(INSEM_HIP_EINVAL, "null argument");
+  *n = c->n_samples;
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_read_traces(insem_hip_ctx* c, double* out) {
+  if (!out) return set_error(INSEM_HIP_EINVAL, "null output");
+  return guarded(c, [&] {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int ns = c->n_samples, nc = c->rec.n_cols;
+    std::vector<float> h(static_cast<size_t>(ns) * std::max(1, nc));
+    if (ns && nc) HIP_TRY(hipMemcpy(h.data(), c->traces, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    const double kh = 0.70710678118654752440;
+    size_t o = 0;
+    for (const TraceCol& tc : c->cols) {
+      for (int s = 0; s < ns; ++s) {
+        const float* row = h.data() + static_cast<size_t>(s) * nc + tc.col;
+        if (tc.m1_elastic && tc.raw_ncomp == 3) {
+          const double up = row[0], um = row[1];
+          out[o++] = kh * (up + um);
+          out[o++] = kh * (up - um);
+          out[o++] = row[2];
+        } else {
+          for (int k = 0; k < tc.raw_ncomp; ++k) out[o++] = row[k];
+        }
+      }
+    }
+  });
+}
+
+int insem_hip_n_energy(insem_hip_ctx* c, int32_t* n) {
+  if (!c || !n) return set_error(INSEM_HIP_EINVAL, "null argument");
+  *n = static_cast<int32_t>(c->energy_steps.size());
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_read_energy(insem_hip_ctx* c, double* out) {
+  if (!out) return set_error(INSEM_HIP_EINVAL, "null output");
+  return guarded(c, [&] {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const size_t n = c->energy_steps.size();
+    std::vector<double> h(2 * std::max<size_t>(1, n));
+    if (n) HIP_TRY(hipMemcpy(h.data(), c->energy_rows, 2 * n * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < n; ++r) {
+      out[3 * r] = c->energy_steps[r] * c->dt;
+      out[3 * r + 1] = h[2 * r + 1];
+      out[3 * r + 2] = -0.5 * h[2 * r];
+    }
+  });
+}
+
+int insem_hip_internal_forces(insem_hip_ctx* c, int32_t di, const float* u, float* f, double* dot) {
+  if (!c || di < 0 || di >= static_cast<int>(c->doms.size())) return set_error(INSEM_HIP_EINVAL, "invalid domain");
+  if (!u || !f) return set_error(INSEM_HIP_EINVAL, "null field");
+  return guarded(c, [&] {
+    DomainRT& R = c->doms[di];
+    std::vector<float> h;
+    permute_in(R, u, h);
+    HIP_TRY(hipMemcpyAsync(R.tmp, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    run_forces(c, R, R.tmp);
+    HIP_TRY(launch_assemble(R.dd, c->stream));
+    if (dot) {
+      int nb = 0;
+      HIP_TRY(launch_dot(R.tmp, R.dd.f, static_cast<int64_t>(R.ncomp) * R.ng, c->partial, &nb, c->stream));
+      HIP_TRY(launch_reduce(c->partial, nb, 1, c->scratch, 0, c->stream));
+      HIP_TRY(hipMemcpyAsync(dot, c->scratch, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(h.data(), R.dd.f, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    permute_out(R, h, f);
+  });
+}
+
+int insem_hip_block_forces(insem_hip_ctx* c, int32_t di, int32_t bi, const float* ul, float* fl) {
+  if (!c || di < 0 || di >= static_cast<int>(c->doms.size())) return set_error(INSEM_HIP_EINVAL, "invalid domain");
+  if (bi < 0 || bi >= static_cast<int>(c->doms[di].blocks.size())) return set_error(INSEM_HIP_EINVAL, "invalid block");
+  if (!ul || !fl) return set_error(INSEM_HIP_EINVAL, "null field");
+  return guarded(c, [&] {
+    DomainRT& R = c->doms[di];
+    BlockRT& br = R.blocks[bi];
+    const size_t n = static_cast<size_t>(R.ne) * R.ncomp * R.npe;
+    float* din = R.fl_tmp;
+    float* dout = c->alloc<float>(n);  // temporary, freed below
+    HIP_TRY(hipMemcpyAsync(din, ul, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(dout, fl, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    ForceArgs a = br.args;
+    a.ul = din;
+    a.fl = dout;
+    a.out_ref = 1;
+    HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, false, a, c->stream));
+    HIP_TRY(hipMemcpyAsync(fl, dout, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->allocs.pop_back();
+    c->footprint -= static_cast<int64_t>(n * sizeof(float));
+    HIP_TRY(hipFree(dout));
+  });
+}
+
+// Power iteration (solver.cpp:583-667) with FP64 reductions on the device.
+int insem_hip_lambda_max(insem_hip_ctx* c, const float* start, double* lambda_out) {
+  if (!start || !lambda_out) return set_error(INSEM_HIP_EINVAL, "null argument");
+  if (c && c->initialized) return set_error(INSEM_HIP_ESTATE, "the power iteration runs before the first step");
+  return guarded(c, [&] {
+    hipStream_t st = c->stream;
+    double* state = c->scratch;       // [0] lambda, [1] done, [2] scale
+    double* out3 = c->scratch + 8;    // reductions
+    double* nxy = c->scratch + 16;    // coupled: [0] nx, [1] ny
+    double best = 0.0;
+    size_t off = 0;
+    for (size_t di = 0; di < c->doms.size(); ++di) {
+      DomainRT& R = c->doms[di];
+      const size_t len = static_cast<size_t>(R.ncomp) * R.ng;
+      std::vector<float> h;
+      permute_in(R, start + off, h);
+      off += len;
+      HIP_TRY(hipMemcpyAsync(R.tmp, h.data(), len * sizeof(float), hipMemcpyHostToDevice, st));
+      const double init[3] = {0.0, 0.0, 0.0};
+      HIP_TRY(hipMemcpyAsync(state, init, sizeof init, hipMemcpyHostToDevice, st));
+      const int all = (1 << R.ncomp) - 1;
+      for (int it = 0; it < 100; ++it) {
+        HIP_TRY(launch_zero_points(R.tmp, R.ng, R.d_dir, R.n_dir, all, R.ncomp, st));
+        HIP_TRY(launch_zero_points(R.tmp, R.ng, R.d_axis, R.n_axis, R.axis_mask, R.ncomp, st));
+        run_forces(c, R, R.tmp);
+        HIP_TRY(launch_assemble(R.dd, st));
+        int nb = 0;
+        HIP_TRY(launch_power_stats(R.dd, R.tmp, c->partial, state, &nb, st));
+        HIP_TRY(launch_reduce(c->partial, nb, 3, out3, 0, st));
+        HIP_TRY(launch_power_update(out3, state, st));
+        HIP_TRY(launch_scale(R.tmp, static_cast<int64_t>(len), state, st));
+      }
+      double lam = 0.0;
+      HIP_TRY(hipMemcpyAsync(&lam, state, sizeof(double), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      best = std::max(best, lam);
+    }
+    if (c->solid >= 0 && c->fluid >= 0 && c->doms[c->solid].n_cpl > 0) {
+      DomainRT& S = c->doms[c->solid];
+      DomainRT& F = c->doms[c->fluid];
+      DomainRT* order[2] = {&S, &F};
+      // Save u and a; start vectors solid u, then fluid u.
+      std::vector<std::vector<float>> saved;
+      for (DomainRT* R : order)
+        for (float* p : {R->dd.u, R->dd.a}) {
+          std::vector<float> h(static_cast<size_t>(R->ncomp) * R->ng);
+          HIP_TRY(hipMemcpy(h.data(), p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+          saved.push_back(std::move(h));
+        }
+      for (DomainRT* R : order) {
+        const size_t len = static_cast<size_t>(R->ncomp) * R->ng;
+        std::vector<float> h;
+        permute_in(*R, start + off, h);
+        off += len;
+        HIP_TRY(hipMemcpyAsync(R->dd.u, h.data(), len * sizeof(float), hipMemcpyHostToDevice, st));
+      }
+      const double init[3] = {0.0, 0.0, 0.0};
+      HIP_TRY(hipMemcpyAsync(state, init, sizeof init, hipMemcpyHostToDevice, st));
+      for (int it = 0; it < 120; ++it) {
+        for (DomainRT* R : order) {
+          HIP_TRY(launch_zero_points(R->dd.u, R->ng, R->d_dir, R->n_dir, (1 << R->ncomp) - 1, R->ncomp, st));
+          HIP_TRY(launch_zero_points(R->dd.u, R->ng, R->d_axis, R->n_axis, R->axis_mask, R->ncomp, st));
+        }
+        for (int q = 0; q < 2; ++q) {
+          int nb = 0;
+          HIP_TRY(launch_mass_norm(order[q]->dd, order[q]->dd.u, c->partial, &nb, st));
+          HIP_TRY(launch_reduce(c->partial, nb, 1, nxy, q, st));
+        }
+        for (DomainRT* R : order) {
+          run_forces(c, *R, R->dd.u);
+          HIP_TRY(launch_assemble(R->dd, st));
+          const DomainRT& O = R == &S ? F : S;
+          HIP_TRY(launch_coupling_power(R->dd, R->cpl_pts, R->cpl_other, R->cpl_cx, R->cpl_cz, R->n_cpl, R->cpl_role,
+                                        R->am == 1, R->cpl_role == 1 ? O.dd.u : O.dd.a, O.ng, st));
+          HIP_TRY(launch_accel(R->dd, st));
+          HIP_TRY(launch_zero_points(R->dd.a, R->ng, R->d_dir, R->n_dir, (1 << R->ncomp) - 1, R->ncomp, st));
+          HIP_TRY(launch_zero_points(R->dd.a, R->ng, R->d_axis, R->n_axis, R->axis_mask, R->ncomp, st));
+        }
+        for (int q = 0; q < 2; ++q) {
+          int nb = 0;
+          HIP_TRY(launch_mass_norm(order[q]->dd, order[q]->dd.a, c->partial, &nb, st));
+          HIP_TRY(launch_reduce(c->partial, nb, 1, nxy + 1, q, st));
+        }
+        HIP_TRY(launch_coupled_update(nxy, nxy + 1, it, state, st));
+        for (DomainRT* R : order)
+          HIP_TRY(launch_scale_from(R->dd.u, R->dd.a, static_cast<int64_t>(R->ncomp) * R->ng, state, st));
+      }
+      double lam = 0.0;
+      HIP_TRY(hipMemcpyAsync(&lam, state, sizeof(double), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      int k = 0;
+      for (DomainRT* R : order)
+        for (float* p : {R->dd.u, R->dd.a}) {
+          HIP_TRY(hipMemcpy(p, saved[k].data(), saved[k].size() * sizeof(float), hipMemcpyHostToDevice));
+          ++k;
+        }
+      best = std::max(best, lam);
+    }
+    *lambda_out = best;
+  });
+}
+
+int insem_hip_synchronize(insem_hip_ctx* c) {
+  return guarded(c, [&] { HIP_TRY(hipStreamSynchronize(c->stream)); });
+}
+
+int insem_hip_set_timing(insem_hip_ctx* c, int32_t enabled) {
+  return guarded(c, [&] {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->timing = enabled ? 1 : 0;
+  });
+}
+
+int insem_hip_kernel_time(insem_hip_ctx* c, double* force_ms, int32_t* force_n, double* update_ms, int32_t* update_n) {
+  return guarded(c, [&] {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->block_ms.clear();
+    for (auto* v : {&c->ev_force, &c->ev_update}) {
+      for (size_t q = 0; q < v->size(); ++q) {
+        auto& e = (*v)[q];
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        if (v == &c->ev_force) {
+          c->force_ms += ms;
+          ++c->force_n;
+          auto& bm = c->block_ms[c->ev_force_tag[q]];
+          bm.first += ms;
+          ++bm.second;
+        } else {
+          c->update_ms += ms;
+          ++c->update_n;
+        }
+        c->ev_pool.push_back(e.first);
+        c->ev_pool.push_back(e.second);
+      }
+      v->clear();
+    }
+    c->ev_force_tag.clear();
+    if (force_ms) *force_ms = c->force_ms;
+    if (force_n) *force_n = c->force_n;
+    if (update_ms) *update_ms = c->update_ms;
+    if (update_n) *update_n = c->update_n;
+    c->force_ms = c->update_ms = 0.0;
+    c->force_n = c->update_n = 0;
+  });
+}
+
+int insem_hip_block_time(insem_hip_ctx* c, int32_t di, int32_t bi, double* ms, int32_t* n) {
+  if (!c || !ms || !n) return set_error(INSEM_HIP_EINVAL, "null argument");
+  const auto it = c->block_ms.find(di * 64 + bi);
+  *ms = it == c->block_ms.end() ? 0.0 : it->second.first;
+  *n = it == c->block_ms.end() ? 0 : it->second.second;
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_footprint(insem_hip_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return set_error(INSEM_HIP_EINVAL, "null argument");
+  *bytes = c->footprint;
+  return INSEM_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// Host-side launchers of the gfx950 kernels.
+#pragma once
+
+#include "device.hpp"
+
+namespace insem_hip {
+
+// Element forces of one block (gather = read the global field through l2g;
+// otherwise read the element-local field in the reference layout).
+hipError_t launch_forces(int ngll, int phys, bool rect, bool gather, const ForceArgs& a, hipStream_t s);
+
+struct StepCoef {
+  float rdt = 0.0f, hdt = 0.0f, hdt2 = 0.0f;  // float(dt), float(dt/2), float(dt^2/2)
+  float h_damp = 0.0f;                        // float(dt/2), or 0 for the explicit damping at t = 0
+  double cc = 0.0;                            // dt^2 / 8 (kinetic energy)
+};
+
+int tail_blocks(int64_t n);
+// Tail of a step on the regular points [0, n_reg): assembly, a = M^-1 f,
+// optional corrector, energy partials ([blocks][2]: u.f, kinetic) and the
+// next step's predictor.
+hipError_t launch_tail_regular(const DomDev& d, const StepCoef& sc, int corrector, int predictor, int energy,
+                               double* partial, int* n_blocks, hipStream_t s);
+// Same on the special points, with sources, coupling, Stacey, Dirichlet and
+// axis conditions; writes the pre-predictor values into the side buffers.
+hipError_t launch_tail_special(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int step, int corrector,
+                               int predictor, int energy, double* partial, int* n_blocks, hipStream_t s);
+// u += rdt v + hdt2 a; v += hdt a over all points.
+hipError_t launch_predictor(const DomDev& d, const StepCoef& sc, hipStream_t s);
+// f = assembly of f_local over all points (no update).
+hipError_t launch_assemble(const DomDev& d, hipStream_t s);
+// Fixed-order sum of n rows of `width` doubles into out (+= when accumulate).
+hipError_t launch_reduce(const double* partial, int n, int width, double* out, int accumulate, hipStream_t s);
+
+struct ReceiverDev {
+  int n_tasks = 0;
+  const int32_t* domain = nullptr;  // per task (receiver quantity): side-buffer slot 0/1
+  const int32_t* kind = nullptr;    // INSEM_Q_*
+  const int32_t* ncomp = nullptr;   // raw device components
+  const int32_t* col = nullptr;     // first column in a trace row
+  const int32_t* pt_off = nullptr;  // [n_tasks + 1]
+  const int32_t* pts = nullptr;     // side-buffer index of each point
+  const float* value = nullptr;
+  const float* d_dx = nullptr;
+  const float* d_dz = nullptr;
+  const float* rho = nullptr;
+  float* pv = nullptr;  // [task][6]: prev_ax, prev_az, vel_x, vel_z, compensation x, z
+  int n_cols = 0;
+  float* traces = nullptr;  // [sample][n_cols]
+  const float* su[2] = {nullptr, nullptr};
+  const float* sv[2] = {nullptr, nullptr};
+  const float* sa[2] = {nullptr, nullptr};
+  int64_t n_sp[2] = {0, 0};
+};
+
+// Records one sample (sample >= 0) and advances the particle velocity
+// (pv_mode 0: take prev from the current field; 1: integrate).
+hipError_t launch_receivers(const ReceiverDev& r, int sample, int pv_mode, float half_dt, hipStream_t s);
+
+// Power iteration pieces (FP64 reductions).  state = {lambda, done, scale}.
+hipError_t launch_zero_points(float* x, int64_t ng, const int32_t* pts, int n, int comp_mask, int ncomp,
+                              hipStream_t s);
+hipError_t launch_power_stats(const DomDev& d, float* x, double* partial, const double* state, int* n_blocks,
+                              hipStream_t s);
+hipError_t launch_power_update(const double* out3, double* state, hipStream_t s);
+hipError_t launch_scale(float* x, int64_t n, const double* state, hipStream_t s);
+hipError_t launch_mass_norm(const DomDev& d, const float* x, double* partial, int* n_blocks, hipStream_t s);
+hipError_t launch_accel(const DomDev& d, hipStream_t s);
+hipError_t launch_coupling_power(const DomDev& d, const int32_t* pts, const int32_t* other, const float* cx,
+                                 const float* cz, int n, int role, int pm, const float* src, int64_t src_ng,
+                                 hipStream_t s);
+hipError_t launch_coupled_update(const double* nx, const double* ny, int it, double* state, hipStream_t s);
+hipError_t launch_scale_from(float* x, const float* a, int64_t n, const double* state, hipStream_t s);
+hipError_t launch_dot(const float* a, const float* b, int64_t n, double* partial, int* n_blocks, hipStream_t s);
+
+}  // namespace insem_hip

@@ -1,0 +1,566 @@
+// Point-wise kernels of the time step on gfx950: deterministic assembly
+// (one work-item per global point, contributions summed in ascending slot
+// order from zero, no atomics), the inverse mass, Stacey, Dirichlet/axis and
+// coupling treatment, the central-difference corrector and predictor,
+// receivers, energy partials and the power-iteration pieces.
+//
+// Reference: cpp/include/insem/ops/dof_map.hpp:122-141 (assembly),
+// cpp/src/solver/solver.cpp:669-762 (sources, coupling, solve),
+// 917-963 (predictor, corrector, particle velocity, record), 900-915
+// (kinetic energy), 583-667 (power iteration).  Float expressions keep the
+// reference's evaluation order; FP64 appears only in the energy and power-
+// iteration reductions.
+
+#include "../../../include/insem_hip.h"
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace insem_hip {
+namespace {
+
+constexpr int kTB = 256;
+
+// Fixed-tree block sum of one double per work-item; result in thread 0.
+__device__ double block_sum(double v, double* lds) {
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int w = kTB / 2; w > 0; w >>= 1) {
+    if (t < w) lds[t] += lds[t + w];
+    __syncthreads();
+  }
+  const double r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+template <int C, bool W>
+__device__ __forceinline__ void gather_sum(const DomDev& d, int64_t g, float* f) {
+#pragma unroll
+  for (int c = 0; c < C; ++c) f[c] = 0.0f;
+  const int32_t s0 = d.t_off[g], s1 = d.t_off[g + 1];
+  for (int32_t s = s0; s < s1; ++s) {
+    const float* p = d.fl + d.t_addr[s];
+    if constexpr (W) {
+      const float w = d.t_w[s];
+#pragma unroll
+      for (int c = 0; c < C; ++c) f[c] += w * p[c * d.npe];
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) f[c] += p[c * d.npe];
+    }
+  }
+}
+
+template <int C, bool W>
+__global__ __launch_bounds__(kTB) void k_tail_regular(DomDev d, StepCoef sc, int corr, int pred, int energy,
+                                                      double* partial) {
+  __shared__ double lds[kTB];
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  double ep = 0.0, ek = 0.0;
+  if (g < d.n_reg) {
+    float f[C];
+    gather_sum<C, W>(d, g, f);
+    const float im = d.im[g];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int64_t o = c * d.ng + g;
+      float u = d.u[o], v = d.v[o];
+      const float a = f[c] * im;
+      if (energy) ep += static_cast<double>(u) * static_cast<double>(f[c]);
+      if (corr) v = v + sc.hdt * a;
+      if (energy) {
+        const double vd = v, ad = a;
+        ek += static_cast<double>(d.mass[g]) * (0.5 * vd * vd - sc.cc * ad * ad);
+      }
+      if (pred) {
+        u = u + (sc.rdt * v + sc.hdt2 * a);
+        v = v + sc.hdt * a;
+        d.u[o] = u;
+      }
+      d.v[o] = v;
+      d.a[o] = a;
+    }
+  }
+  if (energy) {
+    const double sp = block_sum(ep, lds);
+    const double sk = block_sum(ek, lds);
+    if (threadIdx.x == 0) {
+      partial[2 * blockIdx.x] = sp;
+      partial[2 * blockIdx.x + 1] = sk;
+    }
+  }
+}
+
+// Small SPD solve without pivoting, in FP32 (reference: solver.cpp:67-80, in double).
+template <int N>
+__device__ __forceinline__ void solve_small(float* A, float* b) {
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const float l = A[i * N + k] / A[k * N + k];
+#pragma unroll
+      for (int j = k; j < N; ++j) A[i * N + j] -= l * A[k * N + j];
+      b[i] -= l * b[k];
+    }
+#pragma unroll
+  for (int k = N - 1; k >= 0; --k) {
+    float s = b[k];
+#pragma unroll
+    for (int j = k + 1; j < N; ++j) s -= A[k * N + j] * b[j];
+    b[k] = s / A[k * N + k];
+  }
+}
+
+template <int C, bool W>
+__global__ __launch_bounds__(kTB) void k_tail_special(DomDev d, SpecialDev sp, StepCoef sc, int step, int corr,
+                                                      int pred, int energy, double* partial) {
+  __shared__ double lds[kTB];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  double ep = 0.0, ek = 0.0;
+  if (i < sp.n_sp) {
+    const int64_t g = d.n_reg + i;
+    float f[C];
+    gather_sum<C, W>(d, g, f);
+    float u[C], v[C], a[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      u[c] = d.u[c * d.ng + g];
+      v[c] = d.v[c * d.ng + g];
+      if (energy) ep += static_cast<double>(u[c]) * static_cast<double>(f[c]);
+    }
+    // Sources, in source order (solver.cpp:679-687).
+    for (int32_t q = sp.src_off[i]; q < sp.src_off[i + 1]; ++q) {
+      const float amp = sp.amps[static_cast<int64_t>(sp.src_id[q]) * sp.amp_stride + step];
+#pragma unroll
+      for (int c = 0; c < C; ++c) f[c] += sp.src_coef[static_cast<int64_t>(q) * C + c] * amp;
+    }
+    // Coupling (solver.cpp:693-727).
+    const int32_t ci = sp.cpl_idx[i];
+    if (ci >= 0) {
+      const float h = 0.70710678118654752440f;
+      const float cx = sp.cpl_cx[ci], cz = sp.cpl_cz[ci];
+      const int64_t o = sp.cpl_other[ci];
+      if (sp.cpl_role == 1) {
+        if constexpr (C >= 2) {
+          const float p = sp.other[o];
+          if constexpr (C == 3) {
+            f[0] += h * cx * p;
+            f[1] += h * cx * p;
+          } else {
+            f[0] += cx * p;
+          }
+          f[C - 1] += cz * p;
+        }
+      } else {
+        const float ar = sp.pm ? h * (sp.other[o] + sp.other[sp.other_ng + o]) : sp.other[o];
+        const float az = sp.other[(sp.pm ? 2 : 1) * sp.other_ng + o];
+        f[0] -= cx * ar + cz * az;
+      }
+    }
+    // a = M^-1 f, Stacey points by (M + h C) a = f - C v (solver.cpp:729-762).
+    const float im = d.im[g];
+#pragma unroll
+    for (int c = 0; c < C; ++c) a[c] = f[c] * im;
+    const int32_t ai = sp.abs_idx[i];
+    if (ai >= 0) {
+      const float* cf = sp.abs_coef + static_cast<int64_t>(ai) * C * C;
+      const float m = d.mass[g];
+      float A[C * C], rhs[C];
+#pragma unroll
+      for (int r = 0; r < C; ++r) {
+        rhs[r] = f[r];
+#pragma unroll
+        for (int s = 0; s < C; ++s) {
+          rhs[r] -= cf[r * C + s] * v[s];
+          A[r * C + s] = sc.h_damp * cf[r * C + s] + (r == s ? m : 0.0f);
+        }
+      }
+      solve_small<C>(A, rhs);
+#pragma unroll
+      for (int c = 0; c < C; ++c) a[c] = rhs[c];
+    }
+    const uint8_t fl = sp.flags[i];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if ((fl & 1) || ((fl & 2) && ((sp.axis_mask >> c) & 1))) a[c] = 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if (corr) v[c] = v[c] + sc.hdt * a[c];
+      if (energy) {
+        const double vd = v[c], ad = a[c];
+        ek += static_cast<double>(d.mass[g]) * (0.5 * vd * vd - sc.cc * ad * ad);
+      }
+      sp.su[c * sp.n_sp + i] = u[c];
+      sp.sv[c * sp.n_sp + i] = v[c];
+      sp.sa[c * sp.n_sp + i] = a[c];
+      const int64_t o = c * d.ng + g;
+      if (pred) {
+        d.u[o] = u[c] + (sc.rdt * v[c] + sc.hdt2 * a[c]);
+        v[c] = v[c] + sc.hdt * a[c];
+      }
+      d.v[o] = v[c];
+      d.a[o] = a[c];
+    }
+  }
+  if (energy) {
+    const double s0 = block_sum(ep, lds);
+    const double s1 = block_sum(ek, lds);
+    if (threadIdx.x == 0) {
+      partial[2 * blockIdx.x] = s0;
+      partial[2 * blockIdx.x + 1] = s1;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kTB) void k_predictor(float* u, float* v, const float* a, int64_t n, float rdt,
+                                                   float hdt, float hdt2) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  if (i >= n) return;
+  const float vi = v[i], ai = a[i];
+  u[i] = u[i] + (rdt * vi + hdt2 * ai);
+  v[i] = vi + hdt * ai;
+}
+
+template <int C, bool W>
+__global__ __launch_bounds__(kTB) void k_assemble(DomDev d) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  if (g >= d.ng) return;
+  float f[C];
+  gather_sum<C, W>(d, g, f);
+#pragma unroll
+  for (int c = 0; c < C; ++c) d.f[c * d.ng + g] = f[c];
+}
+
+// One work-group sums n rows of `width` doubles in a fixed order.
+__global__ __launch_bounds__(kTB) void k_reduce(const double* partial, int n, int width, double* out, int accumulate) {
+  __shared__ double lds[kTB];
+  for (int w = 0; w < width; ++w) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += kTB) s += partial[static_cast<int64_t>(i) * width + w];
+    const double t = block_sum(s, lds);
+    if (threadIdx.x == 0) out[w] = accumulate ? out[w] + t : t;
+  }
+}
+
+// Receivers (solver.cpp:780-811, 945-958): one work-item per receiver
+// quantity, sums over the receiver's points in their order.
+__global__ void k_receivers(ReceiverDev r, int sample, int pv_mode, float half_dt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= r.n_tasks) return;
+  const int dom = r.domain[t];
+  const int kind = r.kind[t];
+  const int64_t nsp = r.n_sp[dom];
+  const int32_t p0 = r.pt_off[t], p1 = r.pt_off[t + 1];
+  float* row = sample >= 0 ? r.traces + static_cast<int64_t>(sample) * r.n_cols + r.col[t] : nullptr;
+  if (kind == INSEM_Q_PARTICLE_VELOCITY) {
+    const float* u = r.su[dom];
+    float gx = 0.0f, gz = 0.0f;
+    for (int32_t q = p0; q < p1; ++q) {
+      gx += r.d_dx[q] * u[r.pts[q]];
+      gz += r.d_dz[q] * u[r.pts[q]];
+    }
+    const float ax = -gx / r.rho[t], az = -gz / r.rho[t];
+    float* pv = r.pv + 6 * t;
+    if (pv_mode == 1) {
+      // Compensated sum of the trapezoid increments.
+      const float ix = half_dt * (pv[0] + ax) - pv[4];
+      const float sx = pv[2] + ix;
+      pv[4] = (sx - pv[2]) - ix;
+      pv[2] = sx;
+      const float iz = half_dt * (pv[1] + az) - pv[5];
+      const float sz = pv[3] + iz;
+      pv[5] = (sz - pv[3]) - iz;
+      pv[3] = sz;
+    }
+    pv[0] = ax;
+    pv[1] = az;
+    if (row) {
+      row[0] = pv[2];
+      row[1] = pv[3];
+    }
+    return;
+  }
+  if (!row) return;
+  const float* field = kind == INSEM_Q_VELOCITY ? r.sv[dom] : kind == INSEM_Q_ACCELERATION ? r.sa[dom] : r.su[dom];
+  for (int c = 0; c < r.ncomp[t]; ++c) {
+    float s = 0.0f;
+    for (int32_t q = p0; q < p1; ++q) s += r.value[q] * field[c * nsp + r.pts[q]];
+    row[c] = s;
+  }
+}
+
+__global__ void k_zero_points(float* x, int64_t ng, const int32_t* pts, int n, int comp_mask, int ncomp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < ncomp; ++c)
+    if ((comp_mask >> c) & 1) x[c * ng + pts[i]] = 0.0f;
+}
+
+// Power-iteration statistics of one domain (solver.cpp:593-612): u.f,
+// sum m x^2, y = -f/m -> x, sum y^2.
+__global__ __launch_bounds__(kTB) void k_power_stats(DomDev d, float* x, double* partial, const double* state) {
+  __shared__ double lds[kTB];
+  if (state[1] != 0.0) return;
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  double uf = 0.0, xmx = 0.0, nrm = 0.0;
+  if (g < d.ng) {
+    const double m = d.mass[g], im = d.im[g];
+    for (int c = 0; c < d.ncomp; ++c) {
+      const int64_t o = c * d.ng + g;
+      const double xv = x[o], fv = d.f[o];
+      uf += xv * fv;
+      xmx += m * xv * xv;
+      const double y = -fv * im;
+      x[o] = static_cast<float>(y);
+      nrm += y * y;
+    }
+  }
+  const double s0 = block_sum(uf, lds), s1 = block_sum(xmx, lds), s2 = block_sum(nrm, lds);
+  if (threadIdx.x == 0) {
+    partial[3 * blockIdx.x] = s0;
+    partial[3 * blockIdx.x + 1] = s1;
+    partial[3 * blockIdx.x + 2] = s2;
+  }
+}
+
+__global__ void k_power_update(const double* out3, double* state) {
+  if (threadIdx.x != 0 || state[1] != 0.0) return;
+  const double xkx = -out3[0], xmx = out3[1], nrm = out3[2];
+  if (xmx > 0.0 && xkx / xmx > state[0]) state[0] = xkx / xmx;
+  if (nrm == 0.0) {
+    state[1] = 1.0;
+    return;
+  }
+  state[2] = 1.0 / sqrt(nrm);
+}
+
+__global__ __launch_bounds__(kTB) void k_scale(float* x, int64_t n, const double* state) {
+  if (state[1] != 0.0) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  if (i < n) x[i] = static_cast<float>(x[i] * state[2]);
+}
+
+__global__ __launch_bounds__(kTB) void k_mass_norm(DomDev d, const float* x, double* partial) {
+  __shared__ double lds[kTB];
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  double s = 0.0;
+  if (g < d.ng) {
+    const double m = d.mass[g];
+    for (int c = 0; c < d.ncomp; ++c) {
+      const double xv = x[c * d.ng + g];
+      s += m * xv * xv;
+    }
+  }
+  const double t = block_sum(s, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kTB) void k_accel(DomDev d) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  if (g >= d.ng) return;
+  const float im = d.im[g];
+  for (int c = 0; c < d.ncomp; ++c) d.a[c * d.ng + g] = d.f[c * d.ng + g] * im;
+}
+
+__global__ void k_coupling_power(DomDev d, const int32_t* pts, const int32_t* other, const float* cx, const float* cz,
+                                 int n, int role, int pm, const float* src, int64_t src_ng) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float h = 0.70710678118654752440f;
+  const int64_t g = pts[i], o = other[i];
+  if (role == 1) {
+    const float p = src[o];
+    if (pm) {
+      d.f[g] += h * cx[i] * p;
+      d.f[d.ng + g] += h * cx[i] * p;
+    } else {
+      d.f[g] += cx[i] * p;
+    }
+    d.f[(pm ? 2 : 1) * d.ng + g] += cz[i] * p;
+  } else {
+    const float ar = pm ? h * (src[o] + src[src_ng + o]) : src[o];
+    d.f[g] -= cx[i] * ar + cz[i] * src[(pm ? 2 : 1) * src_ng + o];
+  }
+}
+
+// Coupled growth (solver.cpp:640-658): nx, ny are mass norms of u and a.
+__global__ void k_coupled_update(const double* nx, const double* ny, int it, double* state) {
+  if (threadIdx.x != 0 || state[1] != 0.0) return;
+  const double a = nx[0], b = ny[0];
+  if (a <= 0.0 || b <= 0.0) {
+    state[1] = 1.0;
+    return;
+  }
+  if (it >= 60 && sqrt(b / a) > state[0]) state[0] = sqrt(b / a);
+  state[2] = -1.0 / sqrt(b);
+}
+
+__global__ __launch_bounds__(kTB) void k_scale_from(float* x, const float* a, int64_t n, const double* state) {
+  if (state[1] != 0.0) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  if (i < n) x[i] = static_cast<float>(a[i] * state[2]);
+}
+
+__global__ __launch_bounds__(kTB) void k_dot(const float* a, const float* b, int64_t n, double* partial) {
+  __shared__ double lds[kTB];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  const double v = i < n ? static_cast<double>(a[i]) * static_cast<double>(b[i]) : 0.0;
+  const double t = block_sum(v, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+inline int blocks_for(int64_t n) { return static_cast<int>((n + kTB - 1) / kTB); }
+
+}  // namespace
+
+int tail_blocks(int64_t n) { return blocks_for(n); }
+
+hipError_t launch_tail_regular(const DomDev& d, const StepCoef& sc, int corr, int pred, int energy, double* partial,
+                               int* n_blocks, hipStream_t s) {
+  const int nb = blocks_for(d.n_reg);
+  if (n_blocks) *n_blocks = nb;
+  if (nb == 0) return hipSuccess;
+  const bool w = d.t_w != nullptr;
+#define INSEM_TAIL(C)                                                                                            \
+  if (w)                                                                                                         \
+    hipLaunchKernelGGL((k_tail_regular<C, true>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, energy, partial); \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_tail_regular<C, false>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, energy, partial);
+  switch (d.ncomp) {
+    case 1: INSEM_TAIL(1) break;
+    case 2: INSEM_TAIL(2) break;
+    case 3: INSEM_TAIL(3) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef INSEM_TAIL
+  return hipGetLastError();
+}
+
+hipError_t launch_tail_special(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int step, int corr,
+                               int pred, int energy, double* partial, int* n_blocks, hipStream_t s) {
+  const int nb = blocks_for(sp.n_sp);
+  if (n_blocks) *n_blocks = nb;
+  if (nb == 0) return hipSuccess;
+  const bool w = d.t_w != nullptr;
+#define INSEM_TAIL(C)                                                                                         \
+  if (w)                                                                                                      \
+    hipLaunchKernelGGL((k_tail_special<C, true>), dim3(nb), dim3(kTB), 0, s, d, sp, sc, step, corr, pred,     \
+                       energy, partial);                                                                      \
+  else                                                                                                        \
+    hipLaunchKernelGGL((k_tail_special<C, false>), dim3(nb), dim3(kTB), 0, s, d, sp, sc, step, corr, pred,    \
+                       energy, partial);
+  switch (d.ncomp) {
+    case 1: INSEM_TAIL(1) break;
+    case 2: INSEM_TAIL(2) break;
+    case 3: INSEM_TAIL(3) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef INSEM_TAIL
+  return hipGetLastError();
+}
+
+hipError_t launch_predictor(const DomDev& d, const StepCoef& sc, hipStream_t s) {
+  const int64_t n = d.ncomp * d.ng;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_predictor, dim3(blocks_for(n)), dim3(kTB), 0, s, d.u, d.v, d.a, n, sc.rdt, sc.hdt, sc.hdt2);
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble(const DomDev& d, hipStream_t s) {
+  const int nb = blocks_for(d.ng);
+  if (nb == 0) return hipSuccess;
+  const bool w = d.t_w != nullptr;
+#define INSEM_ASM(C)                                                           \
+  if (w)                                                                       \
+    hipLaunchKernelGGL((k_assemble<C, true>), dim3(nb), dim3(kTB), 0, s, d);  \
+  else                                                                         \
+    hipLaunchKernelGGL((k_assemble<C, false>), dim3(nb), dim3(kTB), 0, s, d);
+  switch (d.ncomp) {
+    case 1: INSEM_ASM(1) break;
+    case 2: INSEM_ASM(2) break;
+    case 3: INSEM_ASM(3) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef INSEM_ASM
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double* partial, int n, int width, double* out, int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kTB), 0, s, partial, n, width, out, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_receivers(const ReceiverDev& r, int sample, int pv_mode, float half_dt, hipStream_t s) {
+  if (r.n_tasks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_receivers, dim3((r.n_tasks + 63) / 64), dim3(64), 0, s, r, sample, pv_mode, half_dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_points(float* x, int64_t ng, const int32_t* pts, int n, int comp_mask, int ncomp,
+                              hipStream_t s) {
+  if (n == 0 || comp_mask == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_points, dim3((n + 255) / 256), dim3(256), 0, s, x, ng, pts, n, comp_mask, ncomp);
+  return hipGetLastError();
+}
+
+hipError_t launch_power_stats(const DomDev& d, float* x, double* partial, const double* state, int* n_blocks,
+                              hipStream_t s) {
+  const int nb = blocks_for(d.ng);
+  *n_blocks = nb;
+  hipLaunchKernelGGL(k_power_stats, dim3(nb), dim3(kTB), 0, s, d, x, partial, state);
+  return hipGetLastError();
+}
+
+hipError_t launch_power_update(const double* out3, double* state, hipStream_t s) {
+  hipLaunchKernelGGL(k_power_update, dim3(1), dim3(64), 0, s, out3, state);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale(float* x, int64_t n, const double* state, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale, dim3(blocks_for(n)), dim3(kTB), 0, s, x, n, state);
+  return hipGetLastError();
+}
+
+hipError_t launch_mass_norm(const DomDev& d, const float* x, double* partial, int* n_blocks, hipStream_t s) {
+  const int nb = blocks_for(d.ng);
+  *n_blocks = nb;
+  hipLaunchKernelGGL(k_mass_norm, dim3(nb), dim3(kTB), 0, s, d, x, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_accel(const DomDev& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_accel, dim3(blocks_for(d.ng)), dim3(kTB), 0, s, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_coupling_power(const DomDev& d, const int32_t* pts, const int32_t* other, const float* cx,
+                                 const float* cz, int n, int role, int pm, const float* src, int64_t src_ng,
+                                 hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_coupling_power, dim3((n + 255) / 256), dim3(256), 0, s, d, pts, other, cx, cz, n, role, pm,
+                     src, src_ng);
+  return hipGetLastError();
+}
+
+hipError_t launch_coupled_update(const double* nx, const double* ny, int it, double* state, hipStream_t s) {
+  hipLaunchKernelGGL(k_coupled_update, dim3(1), dim3(64), 0, s, nx, ny, it, state);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_from(float* x, const float* a, int64_t n, const double* state, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_from, dim3(blocks_for(n)), dim3(kTB), 0, s, x, a, n, state);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot(const float* a, const float* b, int64_t n, double* partial, int* n_blocks, hipStream_t s) {
+  const int nb = blocks_for(n);
+  *n_blocks = nb;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kTB), 0, s, a, b, n, partial);
+  return hipGetLastError();
+}
+
+}  // namespace insem_hip

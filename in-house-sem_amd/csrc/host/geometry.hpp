@@ -1,0 +1,48 @@
+// Per-point geometry of the leaves and point location, on the host in double.
+//
+// Restates cpp/src/geometry/geometry.cpp:44-109 (inverse metric, Jacobian,
+// Rect / Affine / Curved classification with relative tolerance 1e-10) and
+// cpp/src/solver/locate.cpp:9-75 (Newton inversion of a leaf mapping,
+// Lagrange interpolation weights and physical gradients).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <optional>
+#include <vector>
+
+#include "basis.hpp"
+#include "mesh.hpp"
+
+namespace insem_amd {
+
+enum class Kind : uint8_t { Rect = 0, Affine = 1, Curved = 2, Axisym = 3 };
+
+struct Geometry {
+  int ngll = 0;
+  int32_t n_elements = 0;
+  std::vector<double> x, z, xi_x, xi_z, eta_x, eta_z, jac;  // [leaf][k]
+  std::vector<Kind> kind;                                    // per leaf
+};
+
+std::array<double, 2> leaf_map(const Forest& f, int32_t leaf, double xi, double eta);
+std::array<double, 4> leaf_jacobian(const Forest& f, int32_t leaf, double xi, double eta);
+
+// axial_xi: the xi points of axial leaves (GLJ) of an axisymmetric forest.
+Geometry compute_geometry(const Forest& f, const std::vector<double>& ref, const std::vector<double>& axial_xi);
+
+struct Location {
+  int32_t leaf = -1;
+  double xi = 0.0, eta = 0.0;
+};
+
+std::optional<Location> locate(const Forest& f, const Geometry& g, std::array<double, 2> x,
+                               const std::vector<int32_t>& candidates);
+
+struct PointWeights {
+  std::vector<double> value, d_dx, d_dz;  // [j * ngll + i]
+};
+
+PointWeights point_weights(const Forest& f, const Lagrange1D& bx, const Lagrange1D& by, const Location& loc);
+
+}  // namespace insem_amd

@@ -1,0 +1,99 @@
+"""Model builders: the reference's test fixtures (tests/cpp/test_helpers.hpp:16-76)
+and the benchmark workloads."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import core as insem
+from . import mesh
+
+
+def grid(nx, nz, x0=0.0, x1=1.0, z0=0.0, z1=1.0, shear=0.0, coords="cartesian", material=None):
+    """nx x nz Quad4 cells on [x0, x1] x [z0, z1], tags 1..4 bottom/right/top/left,
+    x += shear * z (test_helpers.hpp:16-46)."""
+    nodes = []
+    for j in range(nz + 1):
+        for i in range(nx + 1):
+            z = z0 + (z1 - z0) * j / nz
+            nodes.append((x0 + (x1 - x0) * i / nx + shear * z, z))
+    cells, mat, bnd = [], [], []
+    node = lambda i, j: i + (nx + 1) * j  # noqa: E731
+    for j in range(nz):
+        for i in range(nx):
+            c = len(cells)
+            cells.append((node(i, j), node(i + 1, j), node(i + 1, j + 1), node(i, j + 1)))
+            mat.append(0)
+            if j == 0:
+                bnd.append((c, 0, 1))
+            if i == nx - 1:
+                bnd.append((c, 1, 2))
+            if j == nz - 1:
+                bnd.append((c, 2, 3))
+            if i == 0:
+                bnd.append((c, 3, 4))
+    if material is not None:
+        mat = list(material)
+    return insem.MacroMesh("quad4", coords, np.array(nodes), np.array(cells), np.array(mat), np.array(bnd))
+
+
+def quarter_annulus(nr, nt, r0=1.0, r1=2.0):
+    """Quad9 quarter annulus with nodes on circles (test_helpers.hpp:48-76)."""
+    mr, mt = 2 * nr, 2 * nt
+    nodes = []
+    for j in range(mt + 1):
+        for i in range(mr + 1):
+            r = r0 + (r1 - r0) * i / mr
+            t = 0.5 * np.pi * j / mt
+            nodes.append((r * np.cos(t), r * np.sin(t)))
+    node = lambda i, j: i + (mr + 1) * j  # noqa: E731
+    cells = []
+    for j in range(nt):
+        for i in range(nr):
+            a, b = 2 * i, 2 * j
+            cells.append((node(a, b), node(a + 2, b), node(a + 2, b + 2), node(a, b + 2), node(a + 1, b),
+                          node(a + 2, b + 1), node(a + 1, b + 2), node(a, b + 1), node(a + 1, b + 1)))
+    return insem.MacroMesh("quad9", "cartesian", np.array(nodes), np.array(cells), np.zeros(len(cells)))
+
+
+def cfg0(duration=4e-3):
+    """2D acoustic box, 32x32 quads, order 4, Stacey on all sides, monopole
+    Ricker 400 Hz at the centre, pressure receiver at (1.5, 0) (README.md:85-95)."""
+    m = mesh.rectangle((-4.0, 4.0), (-4.0, 4.0), nx=32, nz=32)
+    sim = insem.Simulation(m, insem.Discretization(order=4))
+    sim.set_material(0, insem.Material.acoustic(vp=1500.0, rho=1000.0))
+    for tag in (mesh.BOTTOM, mesh.RIGHT, mesh.TOP, mesh.LEFT):
+        sim.set_boundary(tag, "absorbing")
+    sim.add_source(insem.Source.monopole(0.0, 0.0, insem.ricker(400.0)))
+    sim.add_receiver("h1", 1.5, 0.0, ["pressure"])
+    sim.set_time(duration=duration)
+    return sim
+
+
+def layered_coupled(nx=800, nz_fluid=80, nz_solid=(160, 160), order=4, duration=1.0, n_receivers=101,
+                    n_steps=0, energy_every=0, width=2000.0):
+    """Coupled acoustic-elastic layered model: 2000 x 1000 m, fluid on top
+    (200 m), two elastic layers separated by a sinusoidal interface
+    (amplitude 20 m, wavelength 500 m); free top, absorbing elsewhere;
+    explosion 50 m below the fluid base, 25 Hz Ricker; receivers recording
+    pressure in the fluid and displacement in the solid."""
+    zs = lambda x: -600.0 + 20.0 * np.sin(2.0 * np.pi * x / 500.0)  # noqa: E731
+    m = mesh.layered((0.0, width), [-1000.0, zs, -200.0, 0.0], [nz_solid[0], nz_solid[1], nz_fluid], [2, 1, 0],
+                     nx=nx)
+    sim = insem.Simulation(m, insem.Discretization(order=order))
+    sim.set_material(0, insem.Material.acoustic(1500.0, 1000.0))
+    sim.set_material(1, insem.Material.elastic(2500.0, 1200.0, 2200.0))
+    sim.set_material(2, insem.Material.elastic(4000.0, 2300.0, 2500.0))
+    sim.set_boundary(mesh.TOP, "free")
+    for tag in (mesh.BOTTOM, mesh.LEFT, mesh.RIGHT):
+        sim.set_boundary(tag, "absorbing")
+    x0 = 0.5 * width
+    sim.add_source(insem.Source.moment_tensor(x0, -250.0, 1e6, 1e6, 0.0, insem.ricker(25.0)))
+    xs = np.linspace(0.05 * width, 0.95 * width, n_receivers)
+    for i, x in enumerate(xs):
+        if i % 2 == 0:
+            sim.add_receiver(f"p{i}", x, -100.0, ["pressure"])
+        else:
+            sim.add_receiver(f"s{i}", x, -400.0, ["displacement"])
+    sim.set_time(duration=duration, n_steps=n_steps, energy_every=energy_every)
+    return sim

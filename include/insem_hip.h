@@ -1,0 +1,272 @@
+/*
+ * insem_hip.h -- C-ABI of the MI355X (gfx950) backend for the explicit
+ * time-stepping hot path of insem.
+ *
+ * The boundary is plain C: pointers, sizes and status codes; no C++ or torch
+ * types cross it.  Every entry point returns INSEM_HIP_OK (0) or a negative
+ * status; the message of the last failure on the calling thread is returned
+ * by insem_hip_last_error().  A C++ caller rethrows it as insem::Error
+ * (reference: cpp/include/insem/core/error.hpp:9-25), Python as InsemError.
+ *
+ * Data handed to insem_hip_create() is copied; the caller keeps ownership of
+ * every host buffer.  The context owns all device memory and frees it in
+ * insem_hip_destroy().  A context is not thread-safe (one host thread per
+ * context, as the reference's Solver).
+ *
+ * Layouts follow the reference exactly (cpp/include/insem/ops/dof_map.hpp:15-18):
+ *   global field:  [component][global point]
+ *   local field:   [element][component][j * ngll + i]
+ * The backend may store them differently on the device; every read/write
+ * through this ABI uses the layouts above.
+ */
+#ifndef INSEM_HIP_H
+#define INSEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define INSEM_HIP_OK 0
+#define INSEM_HIP_EINVAL (-1)   /* invalid argument or unsupported configuration */
+#define INSEM_HIP_EDEVICE (-2)  /* HIP runtime failure or no device */
+#define INSEM_HIP_ESTATE (-3)   /* call not allowed in the current state */
+#define INSEM_HIP_ENOMEM (-4)   /* device allocation failed */
+
+/* Geometry kinds (reference: cpp/include/insem/geometry/geometry.hpp:17-22). */
+#define INSEM_KIND_RECT 0
+#define INSEM_KIND_AFFINE 1
+#define INSEM_KIND_CURVED 2
+#define INSEM_KIND_AXISYM 3
+
+/* Media (reference: cpp/include/insem/sim/description.hpp:11). */
+#define INSEM_MEDIUM_ACOUSTIC 0
+#define INSEM_MEDIUM_ELASTIC 1
+
+/* Recorded quantities (reference: description.hpp:225-244). */
+#define INSEM_Q_FIELD 0             /* displacement (elastic) or pressure (acoustic): u */
+#define INSEM_Q_VELOCITY 1          /* v */
+#define INSEM_Q_ACCELERATION 2      /* a */
+#define INSEM_Q_PARTICLE_VELOCITY 3 /* acoustic: trapezoid of -grad(p)/rho */
+
+/* Fields for insem_hip_get_field / set_field. */
+#define INSEM_FIELD_U 0
+#define INSEM_FIELD_V 1
+#define INSEM_FIELD_A 2
+#define INSEM_FIELD_F 3
+
+/*
+ * One homogeneous element group (reference: ops::ElementBlock,
+ * cpp/include/insem/ops/element_block.hpp:31-46).
+ *   metric: Rect 3 per element (xi_x, eta_z, jac); Affine 5 per element;
+ *           Curved and Axisym 5 per point (xi_x, xi_z, eta_x, eta_z, jac),
+ *           [block element][k][5].
+ *   coef0/coef1: per point [block element][k]; elastic lambda, mu;
+ *           acoustic 1/rho and unused.
+ *   wr, hoop, r: Axisym only, per point [block element][k].
+ *   axial: Axisym only, per block element.
+ */
+typedef struct insem_hip_block {
+  int32_t kind;
+  int32_t azimuthal_order;
+  int32_t n_elements;
+  const int32_t* elements; /* domain-local element index of each block element */
+  const float* metric;
+  const float* coef0;
+  const float* coef1;
+  const float* wr;
+  const float* hoop;
+  const float* r;
+  const uint8_t* axial;
+} insem_hip_block;
+
+/*
+ * One medium (reference: solver::Domain, cpp/include/insem/solver/solver.hpp:25-66,
+ * with its ops::DofMap, dof_map.hpp:30-92).
+ *   l2g: [element][k]; >= 0 a global point, < 0 the constrained slot of row -1 - l2g.
+ *   rows: constraint rows (row_offsets has n_rows + 1 entries), weights double.
+ *   t_offsets/t_slots(/t_weights): transpose CSR, slots ascending per point;
+ *     t_weights NULL when the map has no constraints (all weights 1).
+ *   mass, inv_mass: lumped diagonal mass and its inverse, [n_global].
+ *   absorbing: Stacey points and their ncomp x ncomp damping matrices (row-major).
+ *   dirichlet: points whose components are all zero; axis points zero the listed
+ *     axis components.
+ */
+typedef struct insem_hip_domain {
+  int32_t medium;
+  int32_t ncomp;
+  int32_t azimuthal_order;
+  int32_t n_elements;
+  int32_t n_global;
+  const int32_t* l2g;
+  int32_t n_rows;
+  const int32_t* row_offsets;
+  const int32_t* row_dofs;
+  const double* row_weights;
+  const int32_t* t_offsets;
+  const int32_t* t_slots;
+  const double* t_weights;
+  const float* mass;
+  const float* inv_mass;
+  int32_t n_absorbing;
+  const int32_t* absorbing_points;
+  const float* absorbing_coef;
+  int32_t n_dirichlet;
+  const int32_t* dirichlet_points;
+  int32_t n_axis;
+  const int32_t* axis_points;
+  int32_t n_axis_components;
+  const int32_t* axis_components;
+  int32_t n_blocks;
+  const insem_hip_block* blocks;
+} insem_hip_domain;
+
+/* Fluid-solid interface (reference: Solver::Coupling, solver.hpp:208-211). */
+typedef struct insem_hip_coupling {
+  int32_t n_points;
+  const int32_t* fluid_points;
+  const int32_t* solid_points;
+  const float* cx;
+  const float* cz;
+} insem_hip_coupling;
+
+/* Point source (reference: Solver::SourceTerm, solver.hpp:155-160); coef is [comp][point]. */
+typedef struct insem_hip_source {
+  int32_t domain;
+  int32_t n_points;
+  const int32_t* points;
+  const float* coef;
+} insem_hip_source;
+
+/* Receiver (reference: Solver::ReceiverTerm, solver.hpp:161-172). */
+typedef struct insem_hip_receiver {
+  int32_t domain;
+  int32_t n_points;
+  const int32_t* points;
+  const float* value;
+  const float* d_dx;
+  const float* d_dz;
+  double rho;
+  int32_t n_quantities;
+  const int32_t* quantities; /* INSEM_Q_* */
+} insem_hip_receiver;
+
+/*
+ * Everything the time loop needs (reference: the state Solver::Solver builds,
+ * cpp/src/solver/solver.cpp:91-145).  Domains are in the reference's order
+ * (acoustic first, then elastic, solver.cpp:156).
+ */
+typedef struct insem_hip_setup {
+  int32_t ngll;
+  const float* D;  /* [i * ngll + l] = l_l'(x_i) on GLL */
+  const float* Dj; /* same on GLJ(0,1) */
+  const float* w2; /* [j * ngll + i] = w_i w_j */
+  int32_t n_domains;
+  const insem_hip_domain* domains;
+  insem_hip_coupling coupling;
+  int32_t n_sources;
+  const insem_hip_source* sources;
+  int32_t n_receivers;
+  const insem_hip_receiver* receivers;
+  int32_t record_every;
+  int32_t energy_every;
+} insem_hip_setup;
+
+typedef struct insem_hip_ctx insem_hip_ctx;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* insem_hip_last_error(void);
+
+/* Number of visible devices (0 without a GPU; never fails). */
+int insem_hip_device_count(void);
+
+/*
+ * Uploads a setup and allocates the fields (zero).  Replaces the CPU state
+ * the reference's Solver constructor builds (solver.cpp:91-145) on the device.
+ */
+int insem_hip_create(const insem_hip_setup* setup, insem_hip_ctx** out);
+int insem_hip_destroy(insem_hip_ctx* ctx);
+
+/*
+ * Time step, step count and source amplitudes.  amplitudes is
+ * [n_sources][n_steps + 1], amplitudes[s][n] = float(stf_s(n * dt)), the value
+ * the reference adds at step n (solver.cpp:681).  Must be called before the
+ * first advance; replaces the dt/n_steps logic of solver.cpp:131-144.
+ */
+int insem_hip_set_time(insem_hip_ctx* ctx, double dt, int32_t n_steps, const float* amplitudes);
+
+/*
+ * lambda_max(M^-1 K) by power iteration on the device (FP64 reductions):
+ * reference Solver::estimate_stable_dt and coupled_lambda_max
+ * (solver.cpp:583-667).  start holds the start vectors in the reference's
+ * draw order: one vector per domain ([ncomp][n_global], domain order), then,
+ * for coupled models, solid u then fluid u.  The fields are restored.
+ */
+int insem_hip_lambda_max(insem_hip_ctx* ctx, const float* start, double* lambda);
+
+/* Initial conditions of one domain ([ncomp][n_global]); before the first step
+ * only (reference: Solver::set_state, solver.cpp:764-778). */
+int insem_hip_set_state(insem_hip_ctx* ctx, int32_t domain, const float* u, const float* v);
+
+/* Reads a global field of one domain ([ncomp][n_global]). */
+int insem_hip_get_field(insem_hip_ctx* ctx, int32_t domain, int32_t field, float* out);
+
+/* Advances n steps, clipped to n_steps (reference: Solver::advance,
+ * solver.cpp:917-963, with initialize 878-898 on first use). */
+int insem_hip_advance(insem_hip_ctx* ctx, int32_t n);
+
+/* Steps done. */
+int insem_hip_step(insem_hip_ctx* ctx, int32_t* step);
+
+/*
+ * Receiver traces.  n_samples = samples recorded so far.  For receiver r and
+ * quantity q the trace is [n_samples][ncomp] doubles, concatenated in
+ * receiver/quantity order.  Elastic azimuthal-order-1 traces hold
+ * (U_r, U_theta, U_z) (reference: Solver::record, solver.cpp:780-811).
+ */
+int insem_hip_n_samples(insem_hip_ctx* ctx, int32_t* n_samples);
+int insem_hip_read_traces(insem_hip_ctx* ctx, double* out);
+
+/* Energy rows [n_rows][3] = (time, kinetic, potential) (solver.cpp:900-915). */
+int insem_hip_n_energy(insem_hip_ctx* ctx, int32_t* n_rows);
+int insem_hip_read_energy(insem_hip_ctx* ctx, double* out);
+
+/*
+ * Operator-level seams for unit parity.
+ * insem_hip_internal_forces: f = gather -> all blocks -> assemble on the
+ *   given global field, returns u.f in double when dot != NULL
+ *   (reference: Solver::internal_forces, solver.cpp:570-581).
+ * insem_hip_block_forces: one block on an element-local field, writing only the
+ *   block's elements (reference: physics::elastic_internal_forces /
+ *   acoustic_internal_forces, cpp/include/insem/physics/kernels.hpp:32-35).
+ *   u_local and f_local are [n_elements][ncomp][npe] of the domain.
+ */
+int insem_hip_internal_forces(insem_hip_ctx* ctx, int32_t domain, const float* u, float* f, double* dot);
+int insem_hip_block_forces(insem_hip_ctx* ctx, int32_t domain, int32_t block, const float* u_local,
+                           float* f_local);
+
+/* Blocks until all work of the context has finished. */
+int insem_hip_synchronize(insem_hip_ctx* ctx);
+
+/*
+ * Kernel timing with HIP events on the context's stream.  When enabled,
+ * every element-force launch is bracketed by events; insem_hip_kernel_time
+ * returns the summed milliseconds and launch count of the element-force
+ * kernels and of the assembly/update kernels since the last reset.
+ */
+int insem_hip_set_timing(insem_hip_ctx* ctx, int32_t enabled);
+int insem_hip_kernel_time(insem_hip_ctx* ctx, double* force_ms, int32_t* force_launches, double* update_ms,
+                          int32_t* update_launches);
+/* Milliseconds and launch count of the element-force launches of one block
+ * accumulated by the last insem_hip_kernel_time call. */
+int insem_hip_block_time(insem_hip_ctx* ctx, int32_t domain, int32_t block, double* ms, int32_t* launches);
+
+/* Device bytes held by the context. */
+int insem_hip_footprint(insem_hip_ctx* ctx, int64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INSEM_HIP_H */

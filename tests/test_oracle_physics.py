@@ -1,0 +1,229 @@
+"""Whole-solver behaviour of the oracle against physics and mathematics
+(restating tests/cpp/test_solver.cpp:55-172, test_axisym.cpp:124-151,
+test_dipole.cpp:145-200, test_refinement.cpp:194-221 and
+tests/python/test_verification.py:28-117 of the reference):
+energy conserved to round-off, Stacey decay, Dirichlet points at rest,
+stability-limit bounds, the integrator identity of recorded u/v/a, the 2D
+Green's function, the power iteration against a dense eigen-solve, and
+thread-count independence.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import analytic
+import insem_amd as insem
+from models import grid
+from oracle_driver import OracleRun
+
+
+def bump(x, z, x0=0.45, z0=0.55, w=0.08):
+    return np.exp(-((x - x0) ** 2 + (z - z0) ** 2) / (2 * w * w))
+
+
+def run_with_state(sim, medium, ncomp_field, u_fn, threads=0):
+    img = sim.build_setup()
+    o = OracleRun(img, threads=threads)
+    di = img.domain_index(medium)
+    d = img.domains[di]
+    x, z = img.coordinates(di)
+    u = np.zeros((d["ncomp"], d["n_global"]))
+    u[ncomp_field] = u_fn(x, z)
+    o.set_state(di, u, np.zeros_like(u))
+    o.run()
+    return img, o
+
+
+def total(e):
+    return e[:, 1] + e[:, 2]
+
+
+def test_elastic_energy_conserved_in_closed_box():
+    sim = insem.Simulation(grid(6, 6), insem.Discretization(order=4))
+    sim.set_material(0, insem.Material.elastic(2.0, 1.1, 1.3))
+    sim.set_time(n_steps=400, energy_every=20)
+    _, o = run_with_state(sim, "elastic", 0, bump)
+    e = o.energy()
+    assert e.shape == (21, 3)
+    e0 = total(e)[0]
+    assert e0 > 0
+    np.testing.assert_allclose(total(e), e0, rtol=1e-4)  # test_solver.cpp:71-72 (float build)
+    assert e[:, 1].max() > 0.2 * e0
+
+
+@pytest.mark.parametrize("absorbing", [False, True])
+def test_acoustic_energy_rigid_and_absorbing(absorbing):
+    sim = insem.Simulation(grid(8, 8), insem.Discretization(order=4))
+    sim.set_material(0, insem.Material.acoustic(1.5, 1.0))
+    if absorbing:
+        for tag in (1, 2, 3, 4):
+            sim.set_boundary(tag, "absorbing")
+    sim.set_time(duration=3.0, energy_every=10)
+    _, o = run_with_state(sim, "acoustic", 0, bump)
+    e = total(o.energy())
+    if absorbing:
+        assert e[-1] < 0.01 * e[0]
+    else:
+        np.testing.assert_allclose(e[-1], e[0], rtol=1e-4)
+
+
+def test_rigid_elastic_boundaries_stay_at_rest():
+    sim = insem.Simulation(grid(4, 4), insem.Discretization(order=3))
+    sim.set_material(0, insem.Material.elastic(2.0, 1.0, 1.0))
+    for tag in (1, 2, 3, 4):
+        sim.set_boundary(tag, "rigid")
+    sim.add_source(insem.Source.force(0.5, 0.5, 1.0, 1.0, insem.ricker(2.0)))
+    sim.set_time(n_steps=100)
+    img = sim.build_setup()
+    assert img.domains[0]["n_dirichlet"] > 0 and img.domains[0]["n_absorbing"] == 0
+    o = OracleRun(img)
+    o.run()
+    u = o.field(0, 0)
+    x, z = img.coordinates(0)
+    edge = (np.abs(x) < 1e-12) | (np.abs(x - 1) < 1e-12) | (np.abs(z) < 1e-12) | (np.abs(z - 1) < 1e-12)
+    assert np.all(u[:, edge] == 0)
+    assert np.abs(u[0]).max() > 0
+
+
+def test_stability_limit_bounds():
+    sim = insem.Simulation(grid(4, 4), insem.Discretization(order=4))
+    sim.set_material(0, insem.Material.elastic(3.0, 1.5, 1.0))
+    sim.set_time(n_steps=1)
+    o = OracleRun(sim.build_setup())
+    h = 0.25 * (1.0 - np.sqrt(3.0 / 7.0)) / 2.0  # smallest GLL spacing
+    assert 0.2 * h / 3.0 < o.stable_dt < 3.0 * h / 3.0
+
+
+def test_power_iteration_against_dense_eigenvalues():
+    # Tiny elastic model: lambda_max(M^-1 K) from the explicit matrix.
+    sim = insem.Simulation(grid(2, 2), insem.Discretization(order=3))
+    sim.set_material(0, insem.Material.elastic(2.0, 1.2, 1.5))
+    sim.set_time(n_steps=1)
+    img = sim.build_setup()
+    o = OracleRun(img)
+    lam = o.lambda_max()
+    d = img.domains[0]
+    n = d["ncomp"] * d["n_global"]
+    m = np.tile(img.mass(0), d["ncomp"])
+    K = np.zeros((n, n))
+    for k in range(n):
+        e = np.zeros(n, dtype=np.float32)
+        e[k] = 1.0
+        f, _ = o.internal_forces(0, e.reshape(d["ncomp"], -1))
+        K[:, k] = -f.ravel()
+    ev = np.linalg.eigvals(K / m[:, None]).real.max()
+    assert lam <= ev * (1 + 1e-5)
+    assert lam >= 0.95 * ev
+
+
+def test_recorded_fields_obey_the_integrator():
+    # u+ = u + dt v + dt^2/2 a, v+ = v + dt/2 (a + a+) (test_verification.py:103-117)
+    sim = insem.Simulation(grid(4, 4), insem.Discretization(order=4))
+    sim.set_material(0, insem.Material.elastic(2.0, 1.0, 1.0))
+    sim.add_source(insem.Source.force(0.4, 0.45, 1.0, 0.5, insem.ricker(3.0)))
+    sim.add_receiver("r", 0.6, 0.55, ["displacement", "velocity", "acceleration"])
+    sim.set_time(n_steps=200)
+    img = sim.build_setup()
+    o = OracleRun(img)
+    o.run()
+    t = o.traces()
+    u, v, a = (t[("r", q)] for q in ("displacement", "velocity", "acceleration"))
+    dt = o.dt
+    scale_u, scale_v = np.abs(u).max(), np.abs(v).max()
+    np.testing.assert_allclose(u[1:], u[:-1] + dt * v[:-1] + 0.5 * dt * dt * a[:-1], atol=1e-5 * scale_u)
+    np.testing.assert_allclose(v[1:], v[:-1] + 0.5 * dt * (a[:-1] + a[1:]), atol=1e-5 * scale_v)
+
+
+def test_fluid_solid_coupling_runs_and_transmits():
+    m = grid(2, 1, material=[0, 1])
+    sim = insem.Simulation(m, insem.Discretization(order=3, uniform_level=1))
+    sim.set_material(0, insem.Material.acoustic(1.5, 1.0))
+    sim.set_material(1, insem.Material.elastic(3.0, 1.5, 2.0))
+    sim.add_source(insem.Source.monopole(0.25, 0.5, insem.ricker(4.0)))
+    sim.add_receiver("solid side", 0.5, 0.5, ["velocity"], "elastic")
+    sim.add_receiver("fluid side", 0.5, 0.5, ["pressure"], "acoustic")
+    sim.set_time(duration=2.0)
+    img = sim.build_setup()
+    assert img.n_coupling > 0
+    o = OracleRun(img)
+    o.run()
+    t = o.traces()
+    assert np.all(np.isfinite(t[("solid side", "velocity")])) and np.abs(t[("solid side", "velocity")]).max() > 0
+    assert np.abs(t[("fluid side", "pressure")]).max() > 0
+
+
+@pytest.mark.parametrize("medium", ["acoustic", "elastic"])
+@pytest.mark.parametrize("azimuthal_order", [0, 1])
+def test_axisym_energy_conserved_with_hanging_faces(medium, azimuthal_order):
+    m = grid(4, 6, 0.0, 1.0, -1.0, 1.0, coords="axisymmetric")
+    disc = insem.Discretization(order=4, regions=((0.0, 0.3, -0.2, 0.2, 1),), azimuthal_order=azimuthal_order)
+    sim = insem.Simulation(m, disc)
+    sim.set_material(0, insem.Material.elastic(2.0, 1.1, 1.3) if medium == "elastic"
+                     else insem.Material.acoustic(1.5, 1.0))
+    sim.set_time(n_steps=300, energy_every=30)
+    img = sim.build_setup()
+    assert img.domains[0]["constrained"]
+    d = img.domains[0]
+    r, z = img.coordinates(0)
+    u = np.zeros((d["ncomp"], d["n_global"]))
+    u[d["ncomp"] - 1] = np.exp(-(r * r + (z - 0.1) ** 2) / (2 * 0.15 ** 2))
+    if azimuthal_order == 1 and medium == "acoustic":
+        u[0] *= r  # P vanishes on the axis
+    o = OracleRun(img)
+    o.set_state(0, u, np.zeros_like(u))
+    o.run()
+    e = total(o.energy())
+    np.testing.assert_allclose(e, e[0], rtol=1e-4)
+
+
+def test_energy_conserved_on_locally_refined_mesh():
+    disc = insem.Discretization(order=4, uniform_level=1, regions=((0.3, 0.6, 0.4, 0.7, 3),))
+    sim = insem.Simulation(grid(3, 3), disc)
+    sim.set_material(0, insem.Material.elastic(2.0, 1.1, 1.3))
+    sim.set_time(n_steps=300, energy_every=30)
+    assert sim.max_level == 3
+    _, o = run_with_state(sim, "elastic", 0, lambda x, z: bump(x, z, 0.45, 0.55, 0.1))
+    e = total(o.energy())
+    np.testing.assert_allclose(e, e[0], rtol=1e-4)
+
+
+def test_acoustic_monopole_matches_green_function():
+    # test_verification.py:28-47: order 4 at ~6 points per wavelength, 1.3 %
+    c, rho, xr, zr = 1.0, 1.0, 1.5, 0.7
+    m = insem.mesh.rectangle((-4.0, 4.0), (-4.0, 4.0), nx=32, nz=32)
+    sim = insem.Simulation(m, insem.Discretization(order=4))
+    sim.set_material(0, insem.Material.acoustic(c, rho))
+    for tag in (1, 2, 3, 4):
+        sim.set_boundary(tag, "absorbing")
+    stf = insem.ricker(1.0)
+    sim.add_source(insem.Source.monopole(0.0, 0.0, stf))
+    sim.add_receiver("r", xr, zr, ["pressure"])
+    sim.set_time(duration=4.5)
+    img = sim.build_setup()
+    o = OracleRun(img)
+    o.run()
+    p = o.traces()[("r", "pressure")][:, 0]
+    t = np.arange(p.size) * o.dt
+    err = analytic.rel_l2(p, analytic.monopole_pressure(t, stf(t), xr, zr, c, rho))
+    assert err < 0.03
+
+
+def test_runs_are_bitwise_identical_across_thread_counts():
+    sim = insem.Simulation(grid(5, 4, 0, 1, 0, 0.8, 0.1), insem.Discretization(order=4, uniform_level=1))
+    sim.set_material(0, insem.Material.elastic(2.0, 1.0, 1.0))
+    sim.set_boundary(1, "absorbing")
+    sim.set_boundary(2, "absorbing")
+    sim.add_source(insem.Source.moment_tensor(0.42, 0.37, 1.0, 0.5, 0.2, insem.ricker(8.0)))
+    sim.add_receiver("a", 0.7, 0.6, ["displacement", "velocity"])
+    sim.set_time(n_steps=150, record_every=3)
+    traces = []
+    for threads in (1, 8):
+        img = sim.build_setup()
+        o = OracleRun(img, threads=threads)
+        o.run()
+        traces.append(o.traces())
+    for k in traces[0]:
+        assert np.array_equal(traces[0][k], traces[1][k])
+    assert traces[0][("a", "displacement")].shape == (51, 2)
