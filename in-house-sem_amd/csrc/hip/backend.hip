@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -121,13 +122,17 @@ struct insem_hip_ctx {
   int n_steps = 0, step = 0, initialized = 0, have_time = 0;
   float* amps = nullptr;
   int record_every = 1, energy_every = 0;
-  int n_samples = 0, max_samples = 0;
-  float* traces = nullptr;
+  int max_samples = 0;
+  double* traces = nullptr;
+  // step clock and captured step chunks
+  int32_t* d_clock = nullptr;  // steps done, on the device
+  int chunk = 32;
+  int use_graphs = 1;
+  std::map<int, hipGraphExec_t> graphs;
   // energy
   double* partial = nullptr;
   int partial_rows = 0;
   double* energy_rows = nullptr;  // [row][2]: sum u.f, kinetic
-  std::vector<int> energy_steps;
   int max_energy_rows = 0;
   double* scratch = nullptr;  // small reduction outputs
   // timing
@@ -221,6 +226,8 @@ int phys_of(int medium, int kind, int am) {
 }
 
 void free_all(insem_hip_ctx* c) {
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  c->graphs.clear();
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   for (auto& e : c->ev_force) {
@@ -547,7 +554,8 @@ void build_coupling(insem_hip_ctx* c, const insem_hip_setup* s) {
 
 void build_receivers(insem_hip_ctx* c, const insem_hip_setup* s) {
   std::vector<int32_t> dom, kind, nc, col, off{0}, pts;
-  std::vector<float> val, ddx, ddz, rho;
+  std::vector<float> val, ddx, ddz;
+  std::vector<double> rho;
   int ncols = 0;
   for (int ri = 0; ri < s->n_receivers; ++ri) {
     const insem_hip_receiver& r = s->receivers[ri];
@@ -569,7 +577,7 @@ void build_receivers(insem_hip_ctx* c, const insem_hip_setup* s) {
         ddz.push_back(r.d_dz[i]);
       }
       off.push_back(static_cast<int32_t>(pts.size()));
-      rho.push_back(static_cast<float>(r.rho));
+      rho.push_back(r.rho);
     }
   }
   ReceiverDev& rd = c->rec;
@@ -585,8 +593,8 @@ void build_receivers(insem_hip_ctx* c, const insem_hip_setup* s) {
   rd.d_dx = c->upload(ddx);
   rd.d_dz = c->upload(ddz);
   rd.rho = c->upload(rho);
-  rd.pv = c->alloc<float>(6 * static_cast<size_t>(std::max(1, rd.n_tasks)));
-  HIP_TRY(hipMemset(rd.pv, 0, 6 * static_cast<size_t>(std::max(1, rd.n_tasks)) * sizeof(float)));
+  rd.pv = c->alloc<double>(4 * static_cast<size_t>(std::max(1, rd.n_tasks)));
+  HIP_TRY(hipMemset(rd.pv, 0, 4 * static_cast<size_t>(std::max(1, rd.n_tasks)) * sizeof(double)));
   rd.n_cols = ncols;
   for (size_t di = 0; di < c->doms.size(); ++di) {
     rd.su[di] = c->doms[di].sp.su;
@@ -631,14 +639,27 @@ StepCoef step_coef(const insem_hip_ctx* c, bool implicit) {
   sc.rdt = static_cast<float>(c->dt);
   sc.hdt = static_cast<float>(0.5 * c->dt);
   sc.hdt2 = static_cast<float>(0.5 * c->dt * c->dt);
-  sc.h_damp = implicit ? static_cast<float>(0.5 * c->dt) : 0.0f;
+  sc.h_damp = implicit ? 0.5 * c->dt : 0.0;
   sc.cc = c->dt * c->dt / 8.0;
   return sc;
 }
 
-// One compute_acceleration + corrector (+ predictor) over both domains.
-void step_body(insem_hip_ctx* c, int step, bool implicit, int corr, int pred, int energy) {
+StepClock clock_at(const insem_hip_ctx* c, int offset) {
+  StepClock k;
+  k.base = c->d_clock;
+  k.offset = offset;
+  k.energy_every = c->energy_every;
+  k.record_every = c->record_every;
+  k.max_samples = c->max_samples;
+  return k;
+}
+
+// One compute_acceleration + corrector (+ predictor) over both domains, then
+// the receivers, at step *d_clock + offset.  Energy partials are written on
+// energy steps only (the kernels read the step on the device).
+void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, int pv_mode) {
   const StepCoef sc = step_coef(c, implicit);
+  const StepClock clk = clock_at(c, offset);
   int row = 0;
   for (int medium : {INSEM_MEDIUM_ELASTIC, INSEM_MEDIUM_ACOUSTIC}) {
     const int di = medium == INSEM_MEDIUM_ELASTIC ? c->solid : c->fluid;
@@ -648,37 +669,63 @@ void step_body(insem_hip_ctx* c, int step, bool implicit, int corr, int pred, in
     int nb = 0;
     hipEvent_t e0 = nullptr;
     timed_begin(c, &e0);
-    HIP_TRY(launch_tail_regular(R.dd, sc, corr, pred, energy, c->partial + 2 * row, &nb, c->stream));
+    HIP_TRY(launch_tail_regular(R.dd, sc, corr, pred, clk, c->partial + 2 * row, &nb, c->stream));
     timed_end(c, e0, false);
-    row += energy ? nb : 0;
-    HIP_TRY(launch_tail_special(R.dd, R.sp, sc, step, corr, pred, energy, c->partial + 2 * row, &nb, c->stream));
-    row += energy ? nb : 0;
+    row += nb;
+    HIP_TRY(launch_tail_special(R.dd, R.sp, sc, corr, pred, clk, c->partial + 2 * row, &nb, c->stream));
+    row += nb;
   }
-  if (energy) {
-    const int r = static_cast<int>(c->energy_steps.size());
-    HIP_TRY(launch_reduce(c->partial, row, 2, c->energy_rows + 2 * r, 0, c->stream));
-    c->energy_steps.push_back(step);
-  }
-}
-
-void record(insem_hip_ctx* c, int step, int pv_mode) {
-  const bool sample = step % c->record_every == 0;
-  if (!sample && !c->has_pv) return;
-  const int idx = sample ? step / c->record_every : -1;
-  HIP_TRY(launch_receivers(c->rec, idx, pv_mode, static_cast<float>(0.5 * c->dt), c->stream));
-  if (sample) c->n_samples = idx + 1;
+  if (c->energy_every > 0) HIP_TRY(launch_energy_row(c->partial, row, clk, c->energy_rows, c->stream));
+  HIP_TRY(launch_receivers(c->rec, clk, pv_mode, 0.5 * c->dt, c->stream));
 }
 
 void initialize(insem_hip_ctx* c) {
-  const int energy = c->energy_every > 0;
-  step_body(c, 0, false, 0, 0, energy);
-  record(c, 0, 0);
+  step_body(c, 0, false, 0, 0, 0);
   c->initialized = 1;
 }
 
 void predictor_all(insem_hip_ctx* c) {
   const StepCoef sc = step_coef(c, true);
   for (DomainRT& R : c->doms) HIP_TRY(launch_predictor(R.dd, sc, c->stream));
+}
+
+// The launches of k steps after *d_clock: a standalone predictor when the
+// chunk opens an advance call, the predictor of each next step fused into
+// the previous tail, none after the last step of the call.
+void chunk_launches(insem_hip_ctx* c, int k, bool first, bool last) {
+  if (first) predictor_all(c);
+  for (int j = 1; j <= k; ++j) step_body(c, j, true, 1, (last && j == k) ? 0 : 1, 1);
+  HIP_TRY(launch_clock_add(c->d_clock, k, c->stream));
+}
+
+// Runs a chunk, replaying a captured hipGraph of it when graphs are on (the
+// step numbers come from the device clock, so one graph serves every chunk
+// of the same shape).  Timing runs launch directly, with events.
+void run_chunk(insem_hip_ctx* c, int k, bool first, bool last) {
+  if (!c->use_graphs || c->timing) {
+    chunk_launches(c, k, first, last);
+    return;
+  }
+  const int key = (k << 2) | (first ? 2 : 0) | (last ? 1 : 0);
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+      chunk_launches(c, k, first, last);
+    } catch (...) {
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIP_TRY(hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_TRY(e);
+    it = c->graphs.emplace(key, ge).first;
+  }
+  HIP_TRY(hipGraphLaunch(it->second, c->stream));
 }
 
 void require_time(insem_hip_ctx* c) {
@@ -760,6 +807,9 @@ int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) {
     for (const DomainRT& R : c->doms) dot_rows = std::max(dot_rows, tail_blocks(R.ncomp * R.ng));
     c->partial = c->alloc<double>(3 * static_cast<size_t>(std::max(rows, dot_rows)) + 16);
     c->scratch = c->alloc<double>(64);
+    c->d_clock = c->alloc<int32_t>(1);
+    HIP_TRY(hipMemset(c->d_clock, 0, sizeof(int32_t)));
+    if (const char* g = std::getenv("INSEM_HIP_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
     HIP_TRY(hipDeviceSynchronize());
   } catch (const HipFailure& e) {
     free_all(c);
@@ -797,8 +847,8 @@ int insem_hip_set_time(insem_hip_ctx* c, double dt, int32_t n_steps, const float
       R.sp.amp_stride = n_steps + 1;
     }
     c->max_samples = n_steps / c->record_every + 1;
-    c->traces = c->alloc<float>(static_cast<size_t>(c->max_samples) * std::max(1, c->rec.n_cols));
-    HIP_TRY(hipMemset(c->traces, 0, static_cast<size_t>(c->max_samples) * std::max(1, c->rec.n_cols) * sizeof(float)));
+    c->traces = c->alloc<double>(static_cast<size_t>(c->max_samples) * std::max(1, c->rec.n_cols));
+    HIP_TRY(hipMemset(c->traces, 0, static_cast<size_t>(c->max_samples) * std::max(1, c->rec.n_cols) * sizeof(double)));
     c->rec.traces = c->traces;
     c->max_energy_rows = c->energy_every > 0 ? n_steps / c->energy_every + 1 : 0;
     c->energy_rows = c->alloc<double>(2 * static_cast<size_t>(std::max(1, c->max_energy_rows)));
@@ -844,13 +894,11 @@ int insem_hip_advance(insem_hip_ctx* c, int32_t n) {
     require_time(c);
     if (!c->initialized) initialize(c);
     const int m = std::max(0, std::min(n, c->n_steps - c->step));
-    for (int s = 0; s < m; ++s) {
-      if (s == 0) predictor_all(c);
-      ++c->step;
-      const int energy = c->energy_every > 0 && c->step % c->energy_every == 0;
-      step_body(c, c->step, true, 1, s < m - 1 ? 1 : 0, energy);
-      record(c, c->step, 1);
+    for (int s = 0; s < m; s += c->chunk) {
+      const int k = std::min(c->chunk, m - s);
+      run_chunk(c, k, s == 0, s + k == m);
     }
+    c->step += m;
   });
 }
 
@@ -862,7 +910,7 @@ int insem_hip_step(insem_hip_ctx* c, int32_t* step) {
 
 int insem_hip_n_samples(insem_hip_ctx* c, int32_t* n) {
   if (!c || !n) return set_error(INSEM_HIP_EINVAL, "null argument");
-  *n = c->n_samples;
+  *n = c->initialized ? c->step / c->record_every + 1 : 0;
   return INSEM_HIP_OK;
 }
 
@@ -870,14 +918,14 @@ int insem_hip_read_traces(insem_hip_ctx* c, double* out) {
   if (!out) return set_error(INSEM_HIP_EINVAL, "null output");
   return guarded(c, [&] {
     HIP_TRY(hipStreamSynchronize(c->stream));
-    const int ns = c->n_samples, nc = c->rec.n_cols;
-    std::vector<float> h(static_cast<size_t>(ns) * std::max(1, nc));
-    if (ns && nc) HIP_TRY(hipMemcpy(h.data(), c->traces, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    const int ns = c->initialized ? c->step / c->record_every + 1 : 0, nc = c->rec.n_cols;
+    std::vector<double> h(static_cast<size_t>(ns) * std::max(1, nc));
+    if (ns && nc) HIP_TRY(hipMemcpy(h.data(), c->traces, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     const double kh = 0.70710678118654752440;
     size_t o = 0;
     for (const TraceCol& tc : c->cols) {
       for (int s = 0; s < ns; ++s) {
-        const float* row = h.data() + static_cast<size_t>(s) * nc + tc.col;
+        const double* row = h.data() + static_cast<size_t>(s) * nc + tc.col;
         if (tc.m1_elastic && tc.raw_ncomp == 3) {
           const double up = row[0], um = row[1];
           out[o++] = kh * (up + um);
@@ -893,7 +941,7 @@ int insem_hip_read_traces(insem_hip_ctx* c, double* out) {
 
 int insem_hip_n_energy(insem_hip_ctx* c, int32_t* n) {
   if (!c || !n) return set_error(INSEM_HIP_EINVAL, "null argument");
-  *n = static_cast<int32_t>(c->energy_steps.size());
+  *n = c->initialized && c->energy_every > 0 ? c->step / c->energy_every + 1 : 0;
   return INSEM_HIP_OK;
 }
 
@@ -901,11 +949,11 @@ int insem_hip_read_energy(insem_hip_ctx* c, double* out) {
   if (!out) return set_error(INSEM_HIP_EINVAL, "null output");
   return guarded(c, [&] {
     HIP_TRY(hipStreamSynchronize(c->stream));
-    const size_t n = c->energy_steps.size();
+    const size_t n = c->initialized && c->energy_every > 0 ? c->step / c->energy_every + 1 : 0;
     std::vector<double> h(2 * std::max<size_t>(1, n));
     if (n) HIP_TRY(hipMemcpy(h.data(), c->energy_rows, 2 * n * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t r = 0; r < n; ++r) {
-      out[3 * r] = c->energy_steps[r] * c->dt;
+      out[3 * r] = static_cast<double>(r * c->energy_every) * c->dt;
       out[3 * r + 1] = h[2 * r + 1];
       out[3 * r + 2] = -0.5 * h[2 * r];
     }

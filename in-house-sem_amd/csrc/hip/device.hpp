@@ -112,4 +112,19 @@ struct SpecialDev {
   float* sa = nullptr;
 };
 
+// Step number of a launch: *base (the step before the chunk, on the device)
+// + offset (baked into the launch).  Replayed graphs advance base only.
+struct StepClock {
+  const int32_t* base = nullptr;
+  int offset = 0;
+  int energy_every = 0;
+  int record_every = 1;
+  int max_samples = 0;
+};
+
+__device__ __forceinline__ int clock_step(const StepClock& k) { return *k.base + k.offset; }
+__device__ __forceinline__ bool clock_energy(const StepClock& k, int step) {
+  return k.energy_every > 0 && step % k.energy_every == 0;
+}
+
 }  // namespace insem_hip

@@ -11,7 +11,7 @@ hipError_t launch_forces(int ngll, int phys, bool rect, bool gather, const Force
 
 struct StepCoef {
   float rdt = 0.0f, hdt = 0.0f, hdt2 = 0.0f;  // float(dt), float(dt/2), float(dt^2/2)
-  float h_damp = 0.0f;                        // float(dt/2), or 0 for the explicit damping at t = 0
+  double h_damp = 0.0;                        // dt/2, or 0 for the explicit damping at t = 0
   double cc = 0.0;                            // dt^2 / 8 (kinetic energy)
 };
 
@@ -19,12 +19,17 @@ int tail_blocks(int64_t n);
 // Tail of a step on the regular points [0, n_reg): assembly, a = M^-1 f,
 // optional corrector, energy partials ([blocks][2]: u.f, kinetic) and the
 // next step's predictor.
-hipError_t launch_tail_regular(const DomDev& d, const StepCoef& sc, int corrector, int predictor, int energy,
-                               double* partial, int* n_blocks, hipStream_t s);
+// The step number and whether it is an energy step come from the clock.
+hipError_t launch_tail_regular(const DomDev& d, const StepCoef& sc, int corrector, int predictor,
+                               const StepClock& clk, double* partial, int* n_blocks, hipStream_t s);
 // Same on the special points, with sources, coupling, Stacey, Dirichlet and
 // axis conditions; writes the pre-predictor values into the side buffers.
-hipError_t launch_tail_special(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int step, int corrector,
-                               int predictor, int energy, double* partial, int* n_blocks, hipStream_t s);
+hipError_t launch_tail_special(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corrector,
+                               int predictor, const StepClock& clk, double* partial, int* n_blocks, hipStream_t s);
+// rows[step / energy_every] = sums of [n][2] partials, on energy steps only.
+hipError_t launch_energy_row(const double* partial, int n, const StepClock& clk, double* rows, hipStream_t s);
+// *base += n (end of a chunk of n steps).
+hipError_t launch_clock_add(int32_t* base, int n, hipStream_t s);
 // u += rdt v + hdt2 a; v += hdt a over all points.
 hipError_t launch_predictor(const DomDev& d, const StepCoef& sc, hipStream_t s);
 // f = assembly of f_local over all points (no update).
@@ -43,19 +48,20 @@ struct ReceiverDev {
   const float* value = nullptr;
   const float* d_dx = nullptr;
   const float* d_dz = nullptr;
-  const float* rho = nullptr;
-  float* pv = nullptr;  // [task][6]: prev_ax, prev_az, vel_x, vel_z, compensation x, z
+  const double* rho = nullptr;
+  double* pv = nullptr;  // [task][4]: prev_ax, prev_az, vel_x, vel_z
   int n_cols = 0;
-  float* traces = nullptr;  // [sample][n_cols]
+  double* traces = nullptr;  // [sample][n_cols]
   const float* su[2] = {nullptr, nullptr};
   const float* sv[2] = {nullptr, nullptr};
   const float* sa[2] = {nullptr, nullptr};
   int64_t n_sp[2] = {0, 0};
 };
 
-// Records one sample (sample >= 0) and advances the particle velocity
-// (pv_mode 0: take prev from the current field; 1: integrate).
-hipError_t launch_receivers(const ReceiverDev& r, int sample, int pv_mode, float half_dt, hipStream_t s);
+// Records the sample of the clock's step (when it is a recording step) and
+// advances the particle velocity (pv_mode 0: take prev from the current
+// field; 1: integrate).
+hipError_t launch_receivers(const ReceiverDev& r, const StepClock& clk, int pv_mode, double half_dt, hipStream_t s);
 
 // Power iteration pieces (FP64 reductions).  state = {lambda, done, scale}.
 hipError_t launch_zero_points(float* x, int64_t ng, const int32_t* pts, int n, int comp_mask, int ncomp,

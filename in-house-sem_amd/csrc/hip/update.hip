@@ -53,9 +53,10 @@ __device__ __forceinline__ void gather_sum(const DomDev& d, int64_t g, float* f)
 }
 
 template <int C, bool W>
-__global__ __launch_bounds__(kTB) void k_tail_regular(DomDev d, StepCoef sc, int corr, int pred, int energy,
+__global__ __launch_bounds__(kTB) void k_tail_regular(DomDev d, StepCoef sc, int corr, int pred, StepClock clk,
                                                       double* partial) {
   __shared__ double lds[kTB];
+  const bool energy = clock_energy(clk, clock_step(clk));
   const int64_t g = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
   double ep = 0.0, ek = 0.0;
   if (g < d.n_reg) {
@@ -74,12 +75,15 @@ __global__ __launch_bounds__(kTB) void k_tail_regular(DomDev d, StepCoef sc, int
         ek += static_cast<double>(d.mass[g]) * (0.5 * vd * vd - sc.cc * ad * ad);
       }
       if (pred) {
+        // The next step recomputes a at regular points before anything
+        // reads it, so it is only stored when the step ends the call.
         u = u + (sc.rdt * v + sc.hdt2 * a);
         v = v + sc.hdt * a;
         d.u[o] = u;
+      } else {
+        d.a[o] = a;
       }
       d.v[o] = v;
-      d.a[o] = a;
     }
   }
   if (energy) {
@@ -92,21 +96,22 @@ __global__ __launch_bounds__(kTB) void k_tail_regular(DomDev d, StepCoef sc, int
   }
 }
 
-// Small SPD solve without pivoting, in FP32 (reference: solver.cpp:67-80, in double).
+// Small SPD solve without pivoting, in double as the reference's
+// (solver.cpp:67-80); it runs at Stacey points only.
 template <int N>
-__device__ __forceinline__ void solve_small(float* A, float* b) {
+__device__ __forceinline__ void solve_small(double* A, double* b) {
 #pragma unroll
   for (int k = 0; k < N; ++k)
 #pragma unroll
     for (int i = k + 1; i < N; ++i) {
-      const float l = A[i * N + k] / A[k * N + k];
+      const double l = A[i * N + k] / A[k * N + k];
 #pragma unroll
       for (int j = k; j < N; ++j) A[i * N + j] -= l * A[k * N + j];
       b[i] -= l * b[k];
     }
 #pragma unroll
   for (int k = N - 1; k >= 0; --k) {
-    float s = b[k];
+    double s = b[k];
 #pragma unroll
     for (int j = k + 1; j < N; ++j) s -= A[k * N + j] * b[j];
     b[k] = s / A[k * N + k];
@@ -114,9 +119,11 @@ __device__ __forceinline__ void solve_small(float* A, float* b) {
 }
 
 template <int C, bool W>
-__global__ __launch_bounds__(kTB) void k_tail_special(DomDev d, SpecialDev sp, StepCoef sc, int step, int corr,
-                                                      int pred, int energy, double* partial) {
+__global__ __launch_bounds__(kTB) void k_tail_special(DomDev d, SpecialDev sp, StepCoef sc, int corr, int pred,
+                                                      StepClock clk, double* partial) {
   __shared__ double lds[kTB];
+  const int step = clock_step(clk);
+  const bool energy = clock_energy(clk, step);
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
   double ep = 0.0, ek = 0.0;
   if (i < sp.n_sp) {
@@ -166,20 +173,20 @@ __global__ __launch_bounds__(kTB) void k_tail_special(DomDev d, SpecialDev sp, S
     const int32_t ai = sp.abs_idx[i];
     if (ai >= 0) {
       const float* cf = sp.abs_coef + static_cast<int64_t>(ai) * C * C;
-      const float m = d.mass[g];
-      float A[C * C], rhs[C];
+      const double m = d.mass[g];
+      double A[C * C], rhs[C];
 #pragma unroll
       for (int r = 0; r < C; ++r) {
         rhs[r] = f[r];
 #pragma unroll
         for (int s = 0; s < C; ++s) {
-          rhs[r] -= cf[r * C + s] * v[s];
-          A[r * C + s] = sc.h_damp * cf[r * C + s] + (r == s ? m : 0.0f);
+          rhs[r] -= static_cast<double>(cf[r * C + s]) * v[s];
+          A[r * C + s] = sc.h_damp * cf[r * C + s] + (r == s ? m : 0.0);
         }
       }
       solve_small<C>(A, rhs);
 #pragma unroll
-      for (int c = 0; c < C; ++c) a[c] = rhs[c];
+      for (int c = 0; c < C; ++c) a[c] = static_cast<float>(rhs[c]);
     }
     const uint8_t fl = sp.flags[i];
 #pragma unroll
@@ -245,35 +252,53 @@ __global__ __launch_bounds__(kTB) void k_reduce(const double* partial, int n, in
   }
 }
 
+// Energy row of an energy step: rows[step / energy_every] = sums of the
+// [n][2] partials (u.f, kinetic).
+__global__ __launch_bounds__(kTB) void k_energy_row(const double* partial, int n, StepClock clk, double* rows) {
+  __shared__ double lds[kTB];
+  const int step = clock_step(clk);
+  if (!clock_energy(clk, step)) return;
+  for (int w = 0; w < 2; ++w) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += kTB) s += partial[2 * static_cast<int64_t>(i) + w];
+    const double t = block_sum(s, lds);
+    if (threadIdx.x == 0) rows[2 * (step / clk.energy_every) + w] = t;
+  }
+}
+
+__global__ void k_clock_add(int32_t* base, int n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *base += n;
+}
+
 // Receivers (solver.cpp:780-811, 945-958): one work-item per receiver
-// quantity, sums over the receiver's points in their order.
-__global__ void k_receivers(ReceiverDev r, int sample, int pv_mode, float half_dt) {
+// quantity, sums over the receiver's points in their order, accumulated in
+// double from the float fields as the reference does; the particle-velocity
+// trapezoid too.  These O(receivers x npe) reductions are the only FP64 of
+// the step besides the energy monitor.
+__global__ void k_receivers(ReceiverDev r, StepClock clk, int pv_mode, double half_dt) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= r.n_tasks) return;
+  const int step = clock_step(clk);
+  const int sample = step % clk.record_every == 0 && step / clk.record_every < clk.max_samples
+                         ? step / clk.record_every
+                         : -1;
   const int dom = r.domain[t];
   const int kind = r.kind[t];
   const int64_t nsp = r.n_sp[dom];
   const int32_t p0 = r.pt_off[t], p1 = r.pt_off[t + 1];
-  float* row = sample >= 0 ? r.traces + static_cast<int64_t>(sample) * r.n_cols + r.col[t] : nullptr;
+  double* row = sample >= 0 ? r.traces + static_cast<int64_t>(sample) * r.n_cols + r.col[t] : nullptr;
   if (kind == INSEM_Q_PARTICLE_VELOCITY) {
     const float* u = r.su[dom];
-    float gx = 0.0f, gz = 0.0f;
+    double gx = 0.0, gz = 0.0;
     for (int32_t q = p0; q < p1; ++q) {
-      gx += r.d_dx[q] * u[r.pts[q]];
-      gz += r.d_dz[q] * u[r.pts[q]];
+      gx += static_cast<double>(r.d_dx[q]) * u[r.pts[q]];
+      gz += static_cast<double>(r.d_dz[q]) * u[r.pts[q]];
     }
-    const float ax = -gx / r.rho[t], az = -gz / r.rho[t];
-    float* pv = r.pv + 6 * t;
+    const double ax = -gx / r.rho[t], az = -gz / r.rho[t];
+    double* pv = r.pv + 4 * t;  // prev_ax, prev_az, vel_x, vel_z
     if (pv_mode == 1) {
-      // Compensated sum of the trapezoid increments.
-      const float ix = half_dt * (pv[0] + ax) - pv[4];
-      const float sx = pv[2] + ix;
-      pv[4] = (sx - pv[2]) - ix;
-      pv[2] = sx;
-      const float iz = half_dt * (pv[1] + az) - pv[5];
-      const float sz = pv[3] + iz;
-      pv[5] = (sz - pv[3]) - iz;
-      pv[3] = sz;
+      pv[2] += half_dt * (pv[0] + ax);
+      pv[3] += half_dt * (pv[1] + az);
     }
     pv[0] = ax;
     pv[1] = az;
@@ -286,8 +311,8 @@ __global__ void k_receivers(ReceiverDev r, int sample, int pv_mode, float half_d
   if (!row) return;
   const float* field = kind == INSEM_Q_VELOCITY ? r.sv[dom] : kind == INSEM_Q_ACCELERATION ? r.sa[dom] : r.su[dom];
   for (int c = 0; c < r.ncomp[t]; ++c) {
-    float s = 0.0f;
-    for (int32_t q = p0; q < p1; ++q) s += r.value[q] * field[c * nsp + r.pts[q]];
+    double s = 0.0;
+    for (int32_t q = p0; q < p1; ++q) s += static_cast<double>(r.value[q]) * field[c * nsp + r.pts[q]];
     row[c] = s;
   }
 }
@@ -418,17 +443,17 @@ inline int blocks_for(int64_t n) { return static_cast<int>((n + kTB - 1) / kTB);
 
 int tail_blocks(int64_t n) { return blocks_for(n); }
 
-hipError_t launch_tail_regular(const DomDev& d, const StepCoef& sc, int corr, int pred, int energy, double* partial,
-                               int* n_blocks, hipStream_t s) {
+hipError_t launch_tail_regular(const DomDev& d, const StepCoef& sc, int corr, int pred, const StepClock& clk,
+                               double* partial, int* n_blocks, hipStream_t s) {
   const int nb = blocks_for(d.n_reg);
   if (n_blocks) *n_blocks = nb;
   if (nb == 0) return hipSuccess;
   const bool w = d.t_w != nullptr;
-#define INSEM_TAIL(C)                                                                                            \
-  if (w)                                                                                                         \
-    hipLaunchKernelGGL((k_tail_regular<C, true>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, energy, partial); \
-  else                                                                                                           \
-    hipLaunchKernelGGL((k_tail_regular<C, false>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, energy, partial);
+#define INSEM_TAIL(C)                                                                                         \
+  if (w)                                                                                                      \
+    hipLaunchKernelGGL((k_tail_regular<C, true>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, clk, partial); \
+  else                                                                                                        \
+    hipLaunchKernelGGL((k_tail_regular<C, false>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, clk, partial);
   switch (d.ncomp) {
     case 1: INSEM_TAIL(1) break;
     case 2: INSEM_TAIL(2) break;
@@ -439,19 +464,17 @@ hipError_t launch_tail_regular(const DomDev& d, const StepCoef& sc, int corr, in
   return hipGetLastError();
 }
 
-hipError_t launch_tail_special(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int step, int corr,
-                               int pred, int energy, double* partial, int* n_blocks, hipStream_t s) {
+hipError_t launch_tail_special(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr, int pred,
+                               const StepClock& clk, double* partial, int* n_blocks, hipStream_t s) {
   const int nb = blocks_for(sp.n_sp);
   if (n_blocks) *n_blocks = nb;
   if (nb == 0) return hipSuccess;
   const bool w = d.t_w != nullptr;
-#define INSEM_TAIL(C)                                                                                         \
-  if (w)                                                                                                      \
-    hipLaunchKernelGGL((k_tail_special<C, true>), dim3(nb), dim3(kTB), 0, s, d, sp, sc, step, corr, pred,     \
-                       energy, partial);                                                                      \
-  else                                                                                                        \
-    hipLaunchKernelGGL((k_tail_special<C, false>), dim3(nb), dim3(kTB), 0, s, d, sp, sc, step, corr, pred,    \
-                       energy, partial);
+#define INSEM_TAIL(C)                                                                                              \
+  if (w)                                                                                                           \
+    hipLaunchKernelGGL((k_tail_special<C, true>), dim3(nb), dim3(kTB), 0, s, d, sp, sc, corr, pred, clk, partial); \
+  else                                                                                                             \
+    hipLaunchKernelGGL((k_tail_special<C, false>), dim3(nb), dim3(kTB), 0, s, d, sp, sc, corr, pred, clk, partial);
   switch (d.ncomp) {
     case 1: INSEM_TAIL(1) break;
     case 2: INSEM_TAIL(2) break;
@@ -493,9 +516,19 @@ hipError_t launch_reduce(const double* partial, int n, int width, double* out, i
   return hipGetLastError();
 }
 
-hipError_t launch_receivers(const ReceiverDev& r, int sample, int pv_mode, float half_dt, hipStream_t s) {
+hipError_t launch_receivers(const ReceiverDev& r, const StepClock& clk, int pv_mode, double half_dt, hipStream_t s) {
   if (r.n_tasks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_receivers, dim3((r.n_tasks + 63) / 64), dim3(64), 0, s, r, sample, pv_mode, half_dt);
+  hipLaunchKernelGGL(k_receivers, dim3((r.n_tasks + 63) / 64), dim3(64), 0, s, r, clk, pv_mode, half_dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_row(const double* partial, int n, const StepClock& clk, double* rows, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_row, dim3(1), dim3(kTB), 0, s, partial, n, clk, rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_clock_add(int32_t* base, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_clock_add, dim3(1), dim3(64), 0, s, base, n);
   return hipGetLastError();
 }
 

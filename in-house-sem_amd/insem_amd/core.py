@@ -312,9 +312,16 @@ class Solver:
         lib = _lib.host()
         handle = image._h
         image._owned = False  # the solver takes the image over
-        self._h = check_ptr(lib.insem_solver_new(handle))
+        self._handle = check_ptr(lib.insem_solver_new(handle))
         image._h = None
-        self.image = SetupImage(lib.insem_solver_image(self._h), owned=False)
+        self.image = SetupImage(lib.insem_solver_image(self._handle), owned=False)
+
+    @property
+    def _h(self):
+        h = getattr(self, "_handle", None)
+        if not h:
+            raise InsemError("the solver has been closed")
+        return h
 
     def _info(self):
         d = np.zeros(2)
@@ -419,9 +426,10 @@ class Solver:
         return out[:n]
 
     def close(self):
-        if getattr(self, "_h", None):
-            _lib.host().insem_solver_free(self._h)
-            self._h = None
+        h = getattr(self, "_handle", None)
+        if h:
+            _lib.host().insem_solver_free(h)
+        self._handle = None
 
     def __del__(self):
         self.close()
@@ -453,9 +461,8 @@ class Simulation:
         self.is_setup = False
 
     def _invalidate(self):
-        solver = getattr(self, "_solver", None)
-        if solver is not None:
-            solver.close()
+        # A changed description needs a new solver; the old one stays valid
+        # for whoever still holds it and frees its device memory when dropped.
         self._solver = None
 
     def set_material(self, region: int, material: Material) -> None:

@@ -68,11 +68,14 @@ def compare_runs(sim, steps=None, state=None):
         o.advance(steps)
     tg, to = s.traces, o.traces()
     assert tg.keys() == to.keys()
-    worst = 0.0
+    errors = {}
     for k in to:
         assert tg[k].shape == to[k].shape, k
         if np.linalg.norm(to[k]) > 0:
-            worst = max(worst, rel_l2(tg[k], to[k]))
+            errors[k] = rel_l2(tg[k], to[k])
+    for k, e in sorted(errors.items(), key=lambda kv: -kv[1])[:5]:
+        print(f"trace {k}: relative L2 {e:.3e}, peak {np.abs(to[k]).max():.3e}")
+    worst = max(errors.values(), default=0.0)
     return worst, s, o, img_o
 
 
@@ -124,6 +127,130 @@ def test_elastic_block_forces_bitwise(shear):
     fl0 = rng.uniform(-1, 1, size=ul.size).astype(np.float32)
     for bi in range(d["n_blocks"]):
         assert np.array_equal(hip_block_forces(s, 0, bi, ul, fl0), o.block_forces(0, bi, ul, fl0))
+
+
+def small_borehole(azimuthal_order, order=4):
+    """Coupled axisymmetric borehole (mud column + elastic formation) with
+    wavelength refinement, so hanging faces and GLJ axial elements appear."""
+    bh = insem.mesh.borehole((-0.6, 0.6), [("mud", 0.1)], 0.5, dz=0.1)
+    disc = insem.Discretization(order=order, f_max=1.2e4, points_per_wavelength=5.0,
+                                azimuthal_order=azimuthal_order)
+    sim = insem.Simulation(bh.mesh, disc)
+    sim.set_material(bh.regions["mud"], insem.Material.acoustic(1500.0, 1000.0))
+    sim.set_material(bh.regions["formation"], insem.Material.elastic(4000.0, 2300.0, 2500.0))
+    for name in ("bottom", "outer", "top"):
+        sim.set_boundary(bh.boundaries[name], "absorbing")
+    if azimuthal_order == 0:
+        sim.add_source(insem.Source.monopole(0.0, 0.0, insem.ricker(8000.0)))
+    else:
+        sim.add_source(insem.Source.dipole(0.0, 0.0, 1.0, 0.0, insem.ricker(4000.0)))
+    for k, z in enumerate((0.1, 0.25, 0.4)):
+        sim.add_receiver(f"p{k}", 0.05, z, ["pressure", "particle_velocity"])
+        sim.add_receiver(f"s{k}", 0.3, z, ["displacement", "velocity"])
+    sim.set_time(duration=4e-4, energy_every=25)
+    return sim
+
+
+@pytest.mark.parametrize("azimuthal_order", [0, 1])
+def test_borehole_axisym_parity(azimuthal_order):
+    sim = small_borehole(azimuthal_order)
+    worst, s, o, img = compare_runs(sim)
+    assert any(d["constrained"] for d in img.domains)
+    assert img.n_coupling > 0
+    assert worst <= TRACE_RTOL, worst
+    rng = np.random.default_rng(5)
+    for di, d in enumerate(img.domains):
+        u = rng.uniform(-1, 1, size=(d["ncomp"], d["n_global"])).astype(np.float32)
+        assert np.array_equal(hip_internal_forces(s, di, u)[0], o.internal_forces(di, u)[0])
+    eg, eo = s.energy, o.energy()
+    assert eg.shape == eo.shape
+    np.testing.assert_allclose(eg[:, 1:], eo[:, 1:], atol=1e-5 * np.abs(eo[:, 1:]).max())
+
+
+def test_borehole_stable_dt_matches_oracle():
+    sim = small_borehole(1)
+    s = sim.solver
+    o = OracleRun(sim.build_setup(), dt=1e-9)
+    assert s.stable_dt == pytest.approx(2.0 / np.sqrt(o.lambda_max()), rel=1e-9)
+
+
+def test_mixed_kind_blocks_parity():
+    # One domain with Rect, Affine and Curved blocks (a Quad9 mesh whose
+    # right column is sheared and whose top row is curved).
+    nx, nz = 3, 3
+    mx, mz = 2 * nx + 1, 2 * nz + 1
+    nodes = []
+    for j in range(mz):
+        for i in range(mx):
+            x, z = i / (mx - 1), j / (mz - 1)
+            if i >= mx - 3:
+                x += 0.15 * z
+            if j == mz - 1:
+                z += 0.05 * np.sin(np.pi * x)
+            nodes.append((x, z))
+    node = lambda i, j: i + mx * j  # noqa: E731
+    cells, bnd = [], []
+    for j in range(nz):
+        for i in range(nx):
+            a, b = 2 * i, 2 * j
+            c = len(cells)
+            cells.append((node(a, b), node(a + 2, b), node(a + 2, b + 2), node(a, b + 2), node(a + 1, b),
+                          node(a + 2, b + 1), node(a + 1, b + 2), node(a, b + 1), node(a + 1, b + 1)))
+            if j == 0:
+                bnd.append((c, 0, 1))
+            if i == nx - 1:
+                bnd.append((c, 1, 2))
+    m = insem.MacroMesh("quad9", "cartesian", np.array(nodes), np.array(cells), np.zeros(len(cells)), np.array(bnd))
+    sim = insem.Simulation(m, insem.Discretization(order=5, uniform_level=2))
+    sim.set_material(0, insem.Material.elastic(2.0, 1.0, 1.2))
+    sim.set_boundary(1, "absorbing")
+    sim.set_boundary(2, "absorbing")
+    sim.add_source(insem.Source.force(0.4, 0.5, 1.0, -0.5, insem.ricker(6.0)))
+    sim.add_receiver("a", 0.7, 0.3, ["displacement", "acceleration"])
+    sim.set_time(n_steps=300, energy_every=50)
+    kinds = sim.geometry_kind_counts()
+    assert min(kinds.values()) > 0, kinds
+    worst, s, o, img = compare_runs(sim)
+    assert worst <= TRACE_RTOL, worst
+    assert img.domains[0]["n_blocks"] == 3
+
+
+def test_cfg0_particle_velocity_and_fields():
+    sim = cfg0(duration=2e-3)
+    sim.add_receiver("pv", -0.7, 1.1, ["pressure", "particle_velocity"])
+    worst, s, o, img = compare_runs(sim)
+    assert worst <= TRACE_RTOL, worst
+    for which in range(3):
+        g = s.state("acoustic")[which]
+        c = o.field(0, which).astype(np.float64)
+        assert rel_l2(g, c) <= 1e-5
+
+
+def test_runs_are_bitwise_reproducible():
+    out = []
+    for _ in range(2):
+        sim = layered_coupled(nx=40, nz_fluid=4, nz_solid=(8, 8), duration=0.03, n_receivers=7)
+        s = sim.solver
+        s.run()
+        out.append(s.traces)
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k])
+
+
+def test_advance_in_chunks_equals_one_call():
+    # The predictor is fused into the previous step's tail inside one call;
+    # splitting a run into calls must not change a bit.
+    ref = layered_coupled(nx=40, nz_fluid=4, nz_solid=(8, 8), duration=0.03, n_receivers=7)
+    chunked = layered_coupled(nx=40, nz_fluid=4, nz_solid=(8, 8), duration=0.03, n_receivers=7)
+    a, b = ref.solver, chunked.solver
+    a.run()
+    while b.step < b.n_steps:
+        b.advance(7)
+    for k, v in a.traces.items():
+        assert np.array_equal(v, b.traces[k])
+    for medium in ("acoustic", "elastic"):
+        for x, y in zip(a.state(medium), b.state(medium)):
+            assert np.array_equal(x, y)
 
 
 def test_layered_coupled_small_parity():
