@@ -8,10 +8,15 @@ elsewhere; explosion source, 101 receivers.  A step is one central-difference
 time step of the whole model (element forces, assembly, boundary terms,
 coupling, sources, update, receivers), with all fields resident in HBM.
 
-Prints one JSON line with the throughput in GLL DOF-steps/s, the roofline of
-the dominant kernel (element forces of the elastic block) measured with HIP
-events over the timed region, and the CPU oracle timed on the host cores on a
-bounded sample of the same workload.
+Prints one JSON line with the throughput in GLL DOF-steps/s (timed over the
+production path: step chunks replayed as HIP graphs), the roofline of the
+dominant kernel (element forces of the largest elastic block) from HIP events
+around each of its launches in a second, directly launched pass of the same
+steps, and the CPU oracle timed on the host cores on a bounded sample of the
+same workload.
+
+`--calibrate-copy` instead runs a plain device-to-device copy of known size
+(the reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters).
 """
 
 from __future__ import annotations
@@ -20,6 +25,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -27,14 +33,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "in-house-sem_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM traffic of the dominant force launch, measured with rocprofv3 --pmc
+# (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of known size).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_elastic_forces_layered_800x400.txt")
 
 
-def algorithmic_bytes(image):
-    """Bytes per step and per force launch of the dominant block, by the
-    count of the survey's measurement plan: fields 6 c ng + inverse mass ng,
-    metric Rect 3 ne | Affine 5 ne | Curved 5 npe ne | Axisym 8 npe ne,
-    coefficients (2 elastic, 1 acoustic) npe ne, l2g npe ne; FP32 values and
-    int32 indices (4 bytes each)."""
+def algorithmic_bytes(image, hip, ctx, check):
+    """Bytes per step and per force launch of the dominant block.
+
+    Step: fields 6 c ng + inverse mass ng, plus every block's data: metric
+    Rect 3 ne | Affine 5 ne | Curved 5 npe ne | Axisym 8 npe ne, coefficients
+    (2 elastic, 1 acoustic) npe ne and l2g npe ne; FP32 values and int32
+    indices, 4 bytes each.
+
+    Force launch of a time step: the block's data, u at the points it gathers
+    (c per point), and per element slot either the update of its owned point
+    (read v and 1/m, write u and v: 3 c + 1 values) or c forces written for
+    the tail's assembly."""
     npe = image.ngll ** 2
     step = 0
     dominant = None
@@ -47,11 +62,29 @@ def algorithmic_bytes(image):
             metric = {"rect": 3 * ne, "affine": 5 * ne, "curved": 5 * npe * ne, "axisym": 8 * npe * ne}[b["kind"]]
             blk = (metric + coef * npe * ne + npe * ne) * 4
             step += blk
-            # the force launch reads the block data and the field, writes c * npe * ne forces
-            launch = blk + c * ng * 4 + c * npe * ne * 4
-            if dominant is None or launch > dominant[0]:
-                dominant = (launch, d["medium"], b["kind"], ne, di, bi)
+            owned, read = ctypes.c_int64(), ctypes.c_int64()
+            check(hip.insem_hip_block_points(ctx, di, bi, ctypes.byref(owned), ctypes.byref(read)))
+            slots = npe * ne
+            launch = blk + c * read.value * 4 + owned.value * (3 * c + 1) * 4 + (slots - owned.value) * c * 4
+            if dominant is None or launch > dominant["bytes"]:
+                dominant = {"bytes": launch, "medium": d["medium"], "kind": b["kind"], "n_elements": ne,
+                            "domain": di, "block": bi, "owned_slots": owned.value, "points_read": read.value}
     return step, dominant
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch from the committed counter summary, when it was
+    measured on this workload ("traffic_bytes_per_launch: N" line)."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            text = fh.read()
+    except OSError:
+        return None
+    w = re.search(r"^workload: (.*)$", text, re.M)
+    t = re.search(r"^traffic_bytes_per_launch: ([0-9.eE+]+)$", text, re.M)
+    if not w or not t or w.group(1).strip() != workload:
+        return None
+    return float(t.group(1))
 
 
 def cpu_baseline(sim, dt, max_seconds=20.0):
@@ -76,6 +109,18 @@ def cpu_baseline(sim, dt, max_seconds=20.0):
                       f"({el:.1f} s, OpenMP oracle, stepping only)"}
 
 
+def calibrate_copy(n_bytes=1 << 30, reps=5):
+    """A plain copy of n_bytes between two device buffers, reps times."""
+    import torch
+
+    x = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n_bytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,11 +128,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--nx", type=int, default=800)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--calibrate-copy", action="store_true", help="run only a 1 GiB device copy (counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every run of this build uses a single MI355X "
               "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     import insem_amd  # noqa: F401
     from insem_amd import _lib
@@ -100,35 +149,41 @@ def main():
     solver = sim.solver
     setup_s = time.perf_counter() - t0
     img = solver.image
-    need = args.warmup + args.steps
+    timing_steps = min(args.steps, 200)
+    need = args.warmup + args.steps + timing_steps
     if solver.n_steps < need:
-        raise SystemExit(f"the model has {solver.n_steps} steps, fewer than warmup + steps = {need}")
+        raise SystemExit(f"the model has {solver.n_steps} steps, fewer than the {need} this run needs")
     hip = _lib.hip()
+    check = _lib.check_hip
     ctx = ctypes.c_void_p(solver.context)
 
+    # Timed region: the production path (step chunks replayed as hipGraphs).
     solver.advance(args.warmup)
-    _lib.check_hip(hip.insem_hip_synchronize(ctx))
-    _lib.check_hip(hip.insem_hip_set_timing(ctx, 1))
-    fm, fn, um, un = ctypes.c_double(), ctypes.c_int32(), ctypes.c_double(), ctypes.c_int32()
-    _lib.check_hip(hip.insem_hip_kernel_time(ctx, ctypes.byref(fm), ctypes.byref(fn), ctypes.byref(um),
-                                             ctypes.byref(un)))  # reset
-    _lib.check_hip(hip.insem_hip_synchronize(ctx))
+    check(hip.insem_hip_synchronize(ctx))
     t0 = time.perf_counter()
     solver.advance(args.steps)
-    _lib.check_hip(hip.insem_hip_synchronize(ctx))
+    check(hip.insem_hip_synchronize(ctx))
     el = time.perf_counter() - t0
-    _lib.check_hip(hip.insem_hip_kernel_time(ctx, ctypes.byref(fm), ctypes.byref(fn), ctypes.byref(um),
-                                             ctypes.byref(un)))
-    _lib.check_hip(hip.insem_hip_set_timing(ctx, 0))
+
+    # Kernel durations: further steps launched directly, each element-force
+    # launch bracketed by HIP events on the solver's stream.
+    check(hip.insem_hip_set_timing(ctx, 1))
+    fm, fn, um, un = ctypes.c_double(), ctypes.c_int32(), ctypes.c_double(), ctypes.c_int32()
+    check(hip.insem_hip_kernel_time(ctx, ctypes.byref(fm), ctypes.byref(fn), ctypes.byref(um), ctypes.byref(un)))
+    solver.advance(timing_steps)
+    check(hip.insem_hip_synchronize(ctx))
+    check(hip.insem_hip_kernel_time(ctx, ctypes.byref(fm), ctypes.byref(fn), ctypes.byref(um), ctypes.byref(un)))
+    check(hip.insem_hip_set_timing(ctx, 0))
 
     dof = img.n_dofs
     value = dof * args.steps / el
-    step_bytes, dominant = algorithmic_bytes(img)
-    force_avg_ms = fm.value / max(fn.value, 1)
+    step_bytes, dom = algorithmic_bytes(img, hip, ctx, check)
     bms, bn = ctypes.c_double(), ctypes.c_int32()
-    _lib.check_hip(hip.insem_hip_block_time(ctx, dominant[4], dominant[5], ctypes.byref(bms), ctypes.byref(bn)))
+    check(hip.insem_hip_block_time(ctx, dom["domain"], dom["block"], ctypes.byref(bms), ctypes.byref(bn)))
     dom_ms = bms.value / max(bn.value, 1)
-    achieved = dominant[0] / (dom_ms * 1e-3) / 1e9
+    achieved = dom["bytes"] / (dom_ms * 1e-3) / 1e9
+    workload = (f"2D coupled acoustic-elastic layered, {nx}x{nzf + 2 * nzs} quads, order 4, "
+                f"curved elastic blocks, 101 receivers")
     result = {
         "metric": "GLL DOF-steps/s at 1 GPU",
         "value": value,
@@ -142,22 +197,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"2D coupled acoustic-elastic layered, {nx}x{nzf + 2 * nzs} quads, order 4, "
-                               f"curved elastic blocks, 101 receivers",
-                   "n_points": img.n_points, "n_dofs": dof, "dt": solver.dt, "setup_s": round(setup_s, 2)},
+        "config": {"workload": workload, "n_points": img.n_points, "n_dofs": dof, "dt": solver.dt,
+                   "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"element forces, {dominant[1]} {dominant[2]} block ({dominant[3]} elements)",
-                     "algorithmic_bytes": dominant[0], "avg_launch_ms": dom_ms,
-                     "launches": bn.value, "force_avg_ms_all_blocks": force_avg_ms,
-                     "update_ms_per_step": um.value / args.steps,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload),
+                     "kernel": f"element forces + owned-point update, {dom['medium']} {dom['kind']} block "
+                               f"({dom['n_elements']} elements)",
+                     "algorithmic_bytes": dom["bytes"], "avg_launch_ms": dom_ms, "launches": bn.value,
+                     "owned_slots": dom["owned_slots"], "points_read": dom["points_read"],
+                     "force_avg_ms_all_blocks": fm.value / max(fn.value, 1),
+                     "tail_ms_per_step": um.value / timing_steps, "timing_steps": timing_steps,
                      "step_algorithmic_bytes": step_bytes,
                      "step_frac": step_bytes / (el / args.steps) / (HBM_PEAK_GBS * 1e9)},
     }
-    if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(sim, solver.dt)
-    else:
-        result["cpu_baseline"] = None
+    result["cpu_baseline"] = None if args.no_cpu else cpu_baseline(sim, solver.dt)
     print(json.dumps(result), flush=True)
 
 

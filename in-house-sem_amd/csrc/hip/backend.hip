@@ -70,13 +70,16 @@ struct BlockRT {
   bool rect = false;
   int nb = 0;
   int pos0 = 0;
+  int64_t n_owned = 0;  // slots whose point the fused launch updates
+  int64_t n_read = 0;   // distinct points the launch gathers
   ForceArgs args;
   int32_t* d_elem = nullptr;
 };
 
 struct DomainRT {
   int medium = 0, ncomp = 1, am = 0, npe = 1;
-  int64_t ng = 0, n_reg = 0, n_sp = 0, ne = 0;
+  int64_t ng = 0, n_int = 0, n_reg = 0, n_sp = 0, ne = 0;
+  int64_t contrib_len = 0;  // floats of the group layout
   std::vector<int32_t> perm_new, perm_old;
   std::vector<uint8_t> h_dir_mask;  // per old point: 1 Dirichlet, 2 axis (host copy for set_state)
   int axis_mask = 0;
@@ -85,6 +88,7 @@ struct DomainRT {
   std::vector<BlockRT> blocks;
   float* tmp = nullptr;  // [ncomp][ng] scratch (seams, power iteration)
   float* fl_tmp = nullptr;
+  float* contrib = nullptr;  // group layout of the grouped points' forces
   int32_t* d_dir = nullptr;
   int n_dir = 0;
   int32_t* d_axis = nullptr;
@@ -128,6 +132,7 @@ struct insem_hip_ctx {
   int32_t* d_clock = nullptr;  // steps done, on the device
   int chunk = 32;
   int use_graphs = 1;
+  int fuse = 1;  // owned points updated inside the force launches
   std::map<int, hipGraphExec_t> graphs;
   // energy
   double* partial = nullptr;
@@ -310,17 +315,82 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
     if (is_solid) mark(s->coupling.solid_points, s->coupling.n_points);
     if (is_fluid) mark(s->coupling.fluid_points, s->coupling.n_points);
   }
+  // Point classes: 0 owned (one unit-weight slot of its own, not special:
+  // that slot's work-item finishes the point's update), 1 grouped (2 ..
+  // kMaxGroups + 1 unit-weight slots of its own, not special: the slots write
+  // their forces into the group layout), 2 other regular points (assembled
+  // from f_local through the transpose CSR), 3 special.
+  std::vector<uint8_t> cls(static_cast<size_t>(d.n_global), 2);
+  std::vector<uint8_t> cnt(static_cast<size_t>(d.n_global), 0);
+  for (int32_t g = 0; g < d.n_global; ++g) {
+    if (special[g]) {
+      cls[g] = 3;
+      continue;
+    }
+    const int32_t n = d.t_offsets[g + 1] - d.t_offsets[g];
+    bool own = n >= 1 && n <= kMaxGroups + 1;
+    for (int32_t t = d.t_offsets[g]; own && t < d.t_offsets[g + 1]; ++t)
+      own = d.l2g[d.t_slots[t]] == g && (!d.t_weights || static_cast<float>(d.t_weights[t]) == 1.0f);
+    if (own) {
+      cls[g] = n == 1 ? 0 : 1;
+      cnt[g] = static_cast<uint8_t>(n);
+    }
+  }
+  // Within a class, points are numbered in the order the element launches
+  // first touch them (blocks, elements, slots), so that the fused update's
+  // accesses and the tails' reads walk memory in step with the elements;
+  // grouped points are ordered by contribution count first.
+  std::vector<int32_t> touch;
+  touch.reserve(d.n_global);
+  {
+    std::vector<uint8_t> seen(static_cast<size_t>(d.n_global), 0);
+    for (int b = 0; b < d.n_blocks; ++b)
+      for (int e = 0; e < d.blocks[b].n_elements; ++e) {
+        const int32_t* row = d.l2g + static_cast<size_t>(d.blocks[b].elements[e]) * npe;
+        for (int k = 0; k < npe; ++k)
+          if (row[k] >= 0 && !seen[row[k]]) {
+            seen[row[k]] = 1;
+            touch.push_back(row[k]);
+          }
+      }
+    for (int32_t g = 0; g < d.n_global; ++g)
+      if (!seen[g]) touch.push_back(g);
+  }
   R.perm_new.assign(d.n_global, -1);
   R.perm_old.clear();
   R.perm_old.reserve(d.n_global);
-  for (int pass = 0; pass < 2; ++pass)
-    for (int32_t g = 0; g < d.n_global; ++g)
-      if (special[g] == pass) {
-        R.perm_new[g] = static_cast<int32_t>(R.perm_old.size());
-        R.perm_old.push_back(g);
-      }
+  auto take = [&](int32_t g) {
+    R.perm_new[g] = static_cast<int32_t>(R.perm_old.size());
+    R.perm_old.push_back(g);
+  };
+  for (int32_t g : touch)
+    if (cls[g] == 0) take(g);
+  R.n_int = static_cast<int64_t>(R.perm_old.size());
+  Groups& G = R.dd.grp;
+  G = Groups();
+  G.begin = R.n_int;
+  int64_t base = 0;
+  for (int n = 2; n <= kMaxGroups + 1; ++n) {
+    const int64_t s0 = static_cast<int64_t>(R.perm_old.size());
+    for (int32_t g : touch)
+      if (cls[g] == 1 && cnt[g] == n) take(g);
+    const int64_t sz = static_cast<int64_t>(R.perm_old.size()) - s0;
+    if (sz == 0) continue;
+    G.count[G.n] = n;
+    G.start[G.n] = s0;
+    G.size[G.n] = sz;
+    G.base[G.n] = base;
+    base += static_cast<int64_t>(C) * n * sz;
+    ++G.n;
+  }
+  G.end = static_cast<int64_t>(R.perm_old.size());
+  R.contrib_len = base;
+  for (int32_t g : touch)
+    if (cls[g] == 2) take(g);
+  for (int32_t g : touch)
+    if (cls[g] == 3) take(g);
   R.n_sp = 0;
-  for (uint8_t v : special) R.n_sp += v;
+  for (uint8_t v : cls) R.n_sp += v == 3;
   R.n_reg = R.ng - R.n_sp;
 
   // Element positions: blocks concatenated.
@@ -349,6 +419,7 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
   R.dd.ncomp = C;
   R.dd.npe = npe;
   R.dd.ng = R.ng;
+  R.dd.n_int = R.n_int;
   R.dd.n_reg = R.n_reg;
   const size_t len = static_cast<size_t>(C) * R.ng;
   R.dd.u = c->alloc<float>(len);
@@ -374,6 +445,15 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
   HIP_TRY(hipMemset(fl, 0, static_cast<size_t>(d.n_elements) * C * npe * sizeof(float)));
   R.dd.fl = fl;
   R.fl_tmp = c->alloc<float>(static_cast<size_t>(d.n_elements) * C * npe);
+  // Group layout and the rank of each slot of a grouped point.
+  R.dd.contrib = R.contrib = c->alloc<float>(static_cast<size_t>(R.contrib_len));
+  std::vector<uint8_t> slot_rank(static_cast<size_t>(d.n_elements) * npe, 0);
+  for (int32_t g = 0; g < d.n_global; ++g) {
+    const int64_t gn = R.perm_new[g];
+    if (gn < R.dd.grp.begin || gn >= R.dd.grp.end) continue;
+    for (int32_t t = d.t_offsets[g]; t < d.t_offsets[g + 1]; ++t)
+      slot_rank[d.t_slots[t]] = static_cast<uint8_t>(t - d.t_offsets[g]);
+  }
 
   // Constraint rows (new numbering, float weights as the reference's gather uses them).
   const int32_t* row_off = nullptr;
@@ -402,10 +482,21 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
     br.rect = B.kind == INSEM_KIND_RECT;
     const int nb = B.n_elements;
     std::vector<int32_t> l2g(static_cast<size_t>(nb) * npe);
+    std::vector<uint8_t> seen(static_cast<size_t>(d.n_global), 0);
     for (int b = 0; b < nb; ++b)
       for (int k = 0; k < npe; ++k) {
         const int32_t m = d.l2g[static_cast<size_t>(B.elements[b]) * npe + k];
         l2g[static_cast<size_t>(b) * npe + k] = m >= 0 ? R.perm_new[m] : m;
+        br.n_owned += m >= 0 && R.perm_new[m] < R.n_int;
+        if (m >= 0) {
+          br.n_read += !seen[m];
+          seen[m] = 1;
+        } else {
+          for (int32_t q = d.row_offsets[-1 - m]; q < d.row_offsets[-m]; ++q) {
+            br.n_read += !seen[d.row_dofs[q]];
+            seen[d.row_dofs[q]] = 1;
+          }
+        }
       }
     ForceArgs& A = br.args;
     A.nb = nb;
@@ -415,6 +506,21 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
     A.row_w = row_w;
     A.u = R.dd.u;
     A.ng = R.ng;
+    A.n_int = R.n_int;
+    A.uw = R.dd.u;
+    A.v = R.dd.v;
+    A.a = R.dd.a;
+    A.im = R.dd.im;
+    A.mass = R.dd.mass;
+    A.grp = R.dd.grp;
+    A.contrib = R.contrib;
+    {
+      std::vector<uint8_t> rk(static_cast<size_t>(nb) * npe);
+      for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < npe; ++k)
+          rk[static_cast<size_t>(b) * npe + k] = slot_rank[static_cast<size_t>(B.elements[b]) * npe + k];
+      A.rank = c->upload(rk);
+    }
     br.d_elem = c->upload(B.elements, static_cast<size_t>(nb));
     A.elem = br.d_elem;
     A.fl = fl;
@@ -620,18 +726,43 @@ void timed_end(insem_hip_ctx* c, hipEvent_t b, bool force) {
   if (force) c->ev_force_tag.push_back(c->cur_tag);
 }
 
-void run_forces(insem_hip_ctx* c, DomainRT& R, const float* u) {
+// Update of the owned points fused into a time step's force launches.
+struct FuseStep {
+  StepCoef sc;
+  int corr = 0, pred = 0;
+  StepClock clk;
+  double* partial = nullptr;  // first energy row of this domain's launches
+};
+
+// Element forces of every block of R from the field u; with `fs`, the owned
+// points are updated in the same launches (partial rows advance by each
+// launch's work-groups; returns the rows used).
+int run_forces(insem_hip_ctx* c, DomainRT& R, const float* u, const FuseStep* fs = nullptr) {
   const int di = static_cast<int>(&R - c->doms.data());
+  int rows = 0;
   for (size_t bi = 0; bi < R.blocks.size(); ++bi) {
     BlockRT& br = R.blocks[bi];
     c->cur_tag = di * 64 + static_cast<int>(bi);
     ForceArgs a = br.args;
     a.u = u;
+    if (fs) {
+      a.fuse = 1;
+      a.rdt = fs->sc.rdt;
+      a.hdt = fs->sc.hdt;
+      a.hdt2 = fs->sc.hdt2;
+      a.cc = fs->sc.cc;
+      a.corr = fs->corr;
+      a.pred = fs->pred;
+      a.clk = fs->clk;
+      a.partial = fs->partial + 2 * rows;
+      rows += force_blocks(c->ngll, br.nb);
+    }
     hipEvent_t e0 = nullptr;
     timed_begin(c, &e0);
     HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, true, a, c->stream));
     timed_end(c, e0, true);
   }
+  return rows;
 }
 
 StepCoef step_coef(const insem_hip_ctx* c, bool implicit) {
@@ -665,12 +796,27 @@ void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, 
     const int di = medium == INSEM_MEDIUM_ELASTIC ? c->solid : c->fluid;
     if (di < 0) continue;
     DomainRT& R = c->doms[di];
-    run_forces(c, R, R.dd.u);
     int nb = 0;
     hipEvent_t e0 = nullptr;
-    timed_begin(c, &e0);
-    HIP_TRY(launch_tail_regular(R.dd, sc, corr, pred, clk, c->partial + 2 * row, &nb, c->stream));
-    timed_end(c, e0, false);
+    if (c->fuse) {
+      FuseStep fs;
+      fs.sc = sc;
+      fs.corr = corr;
+      fs.pred = pred;
+      fs.clk = clk;
+      fs.partial = c->partial + 2 * row;
+      row += run_forces(c, R, R.dd.u, &fs);
+      timed_begin(c, &e0);
+      HIP_TRY(launch_tail_grouped(R.dd, sc, corr, pred, clk, c->partial + 2 * row, &nb, c->stream));
+      row += nb;
+      HIP_TRY(launch_tail_regular(R.dd, R.dd.grp.end, sc, corr, pred, clk, c->partial + 2 * row, &nb, c->stream));
+      timed_end(c, e0, false);
+    } else {
+      run_forces(c, R, R.dd.u);
+      timed_begin(c, &e0);
+      HIP_TRY(launch_tail_regular(R.dd, 0, sc, corr, pred, clk, c->partial + 2 * row, &nb, c->stream));
+      timed_end(c, e0, false);
+    }
     row += nb;
     HIP_TRY(launch_tail_special(R.dd, R.sp, sc, corr, pred, clk, c->partial + 2 * row, &nb, c->stream));
     row += nb;
@@ -801,7 +947,10 @@ int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) {
     c->record_every = s->record_every;
     c->energy_every = s->energy_every;
     int rows = 0;
-    for (const DomainRT& R : c->doms) rows += tail_blocks(std::max(R.n_reg, R.ng)) + tail_blocks(R.n_sp) + 1;
+    for (const DomainRT& R : c->doms) {
+      rows += tail_blocks(R.ng) + tail_blocks(R.n_sp) + 2;  // grouped + other regular + special
+      for (const BlockRT& br : R.blocks) rows += force_blocks(c->ngll, br.nb);
+    }
     c->partial_rows = rows;
     int dot_rows = 0;
     for (const DomainRT& R : c->doms) dot_rows = std::max(dot_rows, tail_blocks(R.ncomp * R.ng));
@@ -810,6 +959,7 @@ int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) {
     c->d_clock = c->alloc<int32_t>(1);
     HIP_TRY(hipMemset(c->d_clock, 0, sizeof(int32_t)));
     if (const char* g = std::getenv("INSEM_HIP_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
+    if (const char* f = std::getenv("INSEM_HIP_FUSE")) c->fuse = std::atoi(f) != 0;
     HIP_TRY(hipDeviceSynchronize());
   } catch (const HipFailure& e) {
     free_all(c);
@@ -1159,6 +1309,16 @@ int insem_hip_block_time(insem_hip_ctx* c, int32_t di, int32_t bi, double* ms, i
   const auto it = c->block_ms.find(di * 64 + bi);
   *ms = it == c->block_ms.end() ? 0.0 : it->second.first;
   *n = it == c->block_ms.end() ? 0 : it->second.second;
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_block_points(insem_hip_ctx* c, int32_t di, int32_t bi, int64_t* owned, int64_t* read) {
+  if (!c || !owned || !read) return set_error(INSEM_HIP_EINVAL, "null argument");
+  if (di < 0 || di >= static_cast<int32_t>(c->doms.size()) || bi < 0 ||
+      bi >= static_cast<int32_t>(c->doms[di].blocks.size()))
+    return set_error(INSEM_HIP_EINVAL, "no block %d of domain %d", bi, di);
+  *owned = c->fuse ? c->doms[di].blocks[bi].n_owned : 0;
+  *read = c->doms[di].blocks[bi].n_read;
   return INSEM_HIP_OK;
 }
 

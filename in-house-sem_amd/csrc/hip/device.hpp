@@ -40,6 +40,56 @@ __device__ __forceinline__ float ld(const PF& f, int b, int k) {
   return f.p[static_cast<size_t>(b) * f.sb + static_cast<size_t>(k) * f.sk];
 }
 
+// Step number of a launch: *base (the step before the chunk, on the device)
+// + offset (baked into the launch).  Replayed graphs advance base only.
+struct StepClock {
+  const int32_t* base = nullptr;
+  int offset = 0;
+  int energy_every = 0;
+  int record_every = 1;
+  int max_samples = 0;
+};
+
+__device__ __forceinline__ int clock_step(const StepClock& k) { return *k.base + k.offset; }
+__device__ __forceinline__ bool clock_energy(const StepClock& k, int step) {
+  return k.energy_every > 0 && step % k.energy_every == 0;
+}
+
+// Grouped points of a domain: regular points whose contributions all come
+// from element slots of their own (no constraint weights), numbered in
+// groups of equal contribution count.  Group q holds the points
+// [start[q], start[q] + size[q]), each with count[q] contributions, stored
+// at contrib[base[q] + (c * count[q] + r) * size[q] + (point - start[q])]
+// for component c and rank r (the slot's place in the point's ascending
+// slot order), so that the tail reads them in sequence.
+constexpr int kMaxGroups = 7;  // contribution counts 2 .. 8
+struct Groups {
+  int n = 0;
+  int count[kMaxGroups] = {};
+  int64_t start[kMaxGroups] = {};
+  int64_t size[kMaxGroups] = {};
+  int64_t base[kMaxGroups] = {};
+  int64_t begin = 0, end = 0;  // [begin, end): all grouped points
+};
+
+// Group of grouped point g: *off = base + (g - start), *n = size, *cnt =
+// count.  Unrolled selects over the (uniform) table: no indexed loads.
+__device__ __forceinline__ void group_locate(const Groups& G, int64_t g, int64_t* off, int64_t* n, int* cnt) {
+  int64_t b = G.base[0], s = G.start[0], z = G.size[0];
+  int k = G.count[0];
+#pragma unroll
+  for (int q = 1; q < kMaxGroups; ++q) {
+    const bool in = q < G.n && g >= G.start[q];
+    b = in ? G.base[q] : b;
+    s = in ? G.start[q] : s;
+    z = in ? G.size[q] : z;
+    k = in ? G.count[q] : k;
+  }
+  *off = b + (g - s);
+  *n = z;
+  *cnt = k;
+}
+
 // Arguments of one element-force launch (one block of one domain).
 struct ForceArgs {
   int nb = 0;
@@ -64,6 +114,28 @@ struct ForceArgs {
   const float* D = nullptr;
   const float* Dj = nullptr;
   const float* w2 = nullptr;
+  // Time step with the update fused in: the points [0, n_int) each belong to
+  // one element slot only (multiplicity 1, unconstrained, no boundary or
+  // source work), so that slot's force is the assembled force and the
+  // work-item finishes the point's update (M^-1, corrector, energy
+  // partials, next predictor) instead of writing f_local.
+  int fuse = 0;
+  int64_t n_int = 0;
+  float* uw = nullptr;
+  float* v = nullptr;
+  float* a = nullptr;
+  const float* im = nullptr;
+  const float* mass = nullptr;
+  float rdt = 0.0f, hdt = 0.0f, hdt2 = 0.0f;
+  double cc = 0.0;
+  int corr = 0, pred = 0;
+  StepClock clk;
+  double* partial = nullptr;  // [work-group][2]: u.f, kinetic (energy steps)
+  // Slots of grouped points write their force into the group layout
+  // instead of f_local (fused steps only).
+  Groups grp;
+  const uint8_t* rank = nullptr;  // [nb][npe]
+  float* contrib = nullptr;
 };
 
 // Global-point data of one domain on the device, in the backend's point
@@ -74,6 +146,7 @@ struct DomDev {
   int ncomp = 1;
   int npe = 1;
   int64_t ng = 0;
+  int64_t n_int = 0;  // [0, n_int): updated by the element kernels in a time step
   int64_t n_reg = 0;
   float* u = nullptr;
   float* v = nullptr;
@@ -85,6 +158,8 @@ struct DomDev {
   const int32_t* t_addr = nullptr;  // f_local address of each contribution
   const float* t_w = nullptr;       // null without constraints
   const float* fl = nullptr;
+  Groups grp;                       // grouped points (fused steps)
+  const float* contrib = nullptr;
 };
 
 // Special-point data of one domain ([0, n_sp) = points n_reg .. ng - 1).
@@ -111,20 +186,5 @@ struct SpecialDev {
   float* sv = nullptr;
   float* sa = nullptr;
 };
-
-// Step number of a launch: *base (the step before the chunk, on the device)
-// + offset (baked into the launch).  Replayed graphs advance base only.
-struct StepClock {
-  const int32_t* base = nullptr;
-  int offset = 0;
-  int energy_every = 0;
-  int record_every = 1;
-  int max_samples = 0;
-};
-
-__device__ __forceinline__ int clock_step(const StepClock& k) { return *k.base + k.offset; }
-__device__ __forceinline__ bool clock_energy(const StepClock& k, int step) {
-  return k.energy_every > 0 && step % k.energy_every == 0;
-}
 
 }  // namespace insem_hip

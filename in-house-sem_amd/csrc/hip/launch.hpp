@@ -7,7 +7,11 @@ namespace insem_hip {
 
 // Element forces of one block (gather = read the global field through l2g;
 // otherwise read the element-local field in the reference layout).
+// With a.fuse (gather only), the points [0, a.n_int) are updated in the
+// kernel and their energy partials go to a.partial[work-group][2].
 hipError_t launch_forces(int ngll, int phys, bool rect, bool gather, const ForceArgs& a, hipStream_t s);
+// Work-groups of one force launch over nb elements.
+int force_blocks(int ngll, int nb);
 
 struct StepCoef {
   float rdt = 0.0f, hdt = 0.0f, hdt2 = 0.0f;  // float(dt), float(dt/2), float(dt^2/2)
@@ -16,11 +20,14 @@ struct StepCoef {
 };
 
 int tail_blocks(int64_t n);
-// Tail of a step on the regular points [0, n_reg): assembly, a = M^-1 f,
-// optional corrector, energy partials ([blocks][2]: u.f, kinetic) and the
-// next step's predictor.
+// Tail of a step on the regular points [first, n_reg): assembly through the
+// transpose CSR, a = M^-1 f, optional corrector, energy partials
+// ([blocks][2]: u.f, kinetic) and the next step's predictor.
 // The step number and whether it is an energy step come from the clock.
-hipError_t launch_tail_regular(const DomDev& d, const StepCoef& sc, int corrector, int predictor,
+hipError_t launch_tail_regular(const DomDev& d, int64_t first, const StepCoef& sc, int corrector, int predictor,
+                               const StepClock& clk, double* partial, int* n_blocks, hipStream_t s);
+// Same on the grouped points [grp.begin, grp.end), summing the group layout.
+hipError_t launch_tail_grouped(const DomDev& d, const StepCoef& sc, int corrector, int predictor,
                                const StepClock& clk, double* partial, int* n_blocks, hipStream_t s);
 // Same on the special points, with sources, coupling, Stacey, Dirichlet and
 // axis conditions; writes the pre-predictor values into the side buffers.

@@ -52,48 +52,87 @@ __device__ __forceinline__ void gather_sum(const DomDev& d, int64_t g, float* f)
   }
 }
 
+// a = M^-1 f, optional corrector, energy terms and the next predictor at a
+// regular point g with assembled force f.
+template <int C>
+__device__ __forceinline__ void update_point(const DomDev& d, const StepCoef& sc, int corr, int pred, bool energy,
+                                             int64_t g, const float* f, double& ep, double& ek) {
+  const float im = d.im[g];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int64_t o = c * d.ng + g;
+    float u = d.u[o], v = d.v[o];
+    const float a = f[c] * im;
+    if (energy) ep += static_cast<double>(u) * static_cast<double>(f[c]);
+    if (corr) v = v + sc.hdt * a;
+    if (energy) {
+      const double vd = v, ad = a;
+      ek += static_cast<double>(d.mass[g]) * (0.5 * vd * vd - sc.cc * ad * ad);
+    }
+    if (pred) {
+      // The next step recomputes a at regular points before anything
+      // reads it, so it is only stored when the step ends the call.
+      u = u + (sc.rdt * v + sc.hdt2 * a);
+      v = v + sc.hdt * a;
+      d.u[o] = u;
+    } else {
+      d.a[o] = a;
+    }
+    d.v[o] = v;
+  }
+}
+
+__device__ __forceinline__ void write_partials(bool energy, double ep, double ek, double* lds, double* partial) {
+  if (!energy) return;
+  const double sp = block_sum(ep, lds);
+  const double sk = block_sum(ek, lds);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = sp;
+    partial[2 * blockIdx.x + 1] = sk;
+  }
+}
+
+// Regular points [first, n_reg) assembled through the transpose CSR.
 template <int C, bool W>
-__global__ __launch_bounds__(kTB) void k_tail_regular(DomDev d, StepCoef sc, int corr, int pred, StepClock clk,
-                                                      double* partial) {
+__global__ __launch_bounds__(kTB) void k_tail_regular(DomDev d, int64_t first, StepCoef sc, int corr, int pred,
+                                                      StepClock clk, double* partial) {
   __shared__ double lds[kTB];
   const bool energy = clock_energy(clk, clock_step(clk));
-  const int64_t g = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  const int64_t g = first + static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
   double ep = 0.0, ek = 0.0;
   if (g < d.n_reg) {
     float f[C];
     gather_sum<C, W>(d, g, f);
-    const float im = d.im[g];
+    update_point<C>(d, sc, corr, pred, energy, g, f, ep, ek);
+  }
+  write_partials(energy, ep, ek, lds, partial);
+}
+
+// Grouped points: the contributions of a point lie in rank order at stride
+// size[q] in the group layout; the sum starts from zero in slot order as the
+// assembly's does.
+template <int C>
+__global__ __launch_bounds__(kTB) void k_tail_grouped(DomDev d, StepCoef sc, int corr, int pred, StepClock clk,
+                                                      double* partial) {
+  __shared__ double lds[kTB];
+  const bool energy = clock_energy(clk, clock_step(clk));
+  const int64_t g = d.grp.begin + static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  double ep = 0.0, ek = 0.0;
+  if (g < d.grp.end) {
+    int64_t off, n;
+    int cnt;
+    group_locate(d.grp, g, &off, &n, &cnt);
+    const float* src = d.contrib + off;
+    float f[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int64_t o = c * d.ng + g;
-      float u = d.u[o], v = d.v[o];
-      const float a = f[c] * im;
-      if (energy) ep += static_cast<double>(u) * static_cast<double>(f[c]);
-      if (corr) v = v + sc.hdt * a;
-      if (energy) {
-        const double vd = v, ad = a;
-        ek += static_cast<double>(d.mass[g]) * (0.5 * vd * vd - sc.cc * ad * ad);
-      }
-      if (pred) {
-        // The next step recomputes a at regular points before anything
-        // reads it, so it is only stored when the step ends the call.
-        u = u + (sc.rdt * v + sc.hdt2 * a);
-        v = v + sc.hdt * a;
-        d.u[o] = u;
-      } else {
-        d.a[o] = a;
-      }
-      d.v[o] = v;
+      float s = 0.0f;
+      for (int r = 0; r < cnt; ++r) s += src[(c * cnt + r) * n];
+      f[c] = s;
     }
+    update_point<C>(d, sc, corr, pred, energy, g, f, ep, ek);
   }
-  if (energy) {
-    const double sp = block_sum(ep, lds);
-    const double sk = block_sum(ek, lds);
-    if (threadIdx.x == 0) {
-      partial[2 * blockIdx.x] = sp;
-      partial[2 * blockIdx.x + 1] = sk;
-    }
-  }
+  write_partials(energy, ep, ek, lds, partial);
 }
 
 // Small SPD solve without pivoting, in double as the reference's
@@ -275,9 +314,34 @@ __global__ void k_clock_add(int32_t* base, int n) {
 // double from the float fields as the reference does; the particle-velocity
 // trapezoid too.  These O(receivers x npe) reductions are the only FP64 of
 // the step besides the energy monitor.
-__global__ void k_receivers(ReceiverDev r, StepClock clk, int pv_mode, double half_dt) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= r.n_tasks) return;
+constexpr int kRecWG = 64;
+
+// Sums w[q] * x[q] over q in [p0, p1) in ascending q, in double: the lanes
+// stage the operands in LDS, lane 0 adds them in order (the reference's
+// serial order, without a chain of dependent global loads).
+__device__ double ordered_dot(const float* w, const float* x, const int32_t* pts, int64_t stride, int32_t p0,
+                              int32_t p1, double* sw, float* sx) {
+  double s = 0.0;
+  for (int32_t base = p0; base < p1; base += kRecWG) {
+    const int32_t q = base + static_cast<int32_t>(threadIdx.x);
+    if (q < p1) {
+      sw[threadIdx.x] = w[q];
+      sx[threadIdx.x] = x[stride + pts[q]];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int32_t n = p1 - base < kRecWG ? p1 - base : kRecWG;
+      for (int32_t i = 0; i < n; ++i) s += sw[i] * static_cast<double>(sx[i]);
+    }
+    __syncthreads();
+  }
+  return s;
+}
+
+__global__ __launch_bounds__(kRecWG) void k_receivers(ReceiverDev r, StepClock clk, int pv_mode, double half_dt) {
+  __shared__ double sw[kRecWG];
+  __shared__ float sx[kRecWG];
+  const int t = blockIdx.x;
   const int step = clock_step(clk);
   const int sample = step % clk.record_every == 0 && step / clk.record_every < clk.max_samples
                          ? step / clk.record_every
@@ -289,11 +353,9 @@ __global__ void k_receivers(ReceiverDev r, StepClock clk, int pv_mode, double ha
   double* row = sample >= 0 ? r.traces + static_cast<int64_t>(sample) * r.n_cols + r.col[t] : nullptr;
   if (kind == INSEM_Q_PARTICLE_VELOCITY) {
     const float* u = r.su[dom];
-    double gx = 0.0, gz = 0.0;
-    for (int32_t q = p0; q < p1; ++q) {
-      gx += static_cast<double>(r.d_dx[q]) * u[r.pts[q]];
-      gz += static_cast<double>(r.d_dz[q]) * u[r.pts[q]];
-    }
+    const double gx = ordered_dot(r.d_dx, u, r.pts, 0, p0, p1, sw, sx);
+    const double gz = ordered_dot(r.d_dz, u, r.pts, 0, p0, p1, sw, sx);
+    if (threadIdx.x != 0) return;
     const double ax = -gx / r.rho[t], az = -gz / r.rho[t];
     double* pv = r.pv + 4 * t;  // prev_ax, prev_az, vel_x, vel_z
     if (pv_mode == 1) {
@@ -311,9 +373,8 @@ __global__ void k_receivers(ReceiverDev r, StepClock clk, int pv_mode, double ha
   if (!row) return;
   const float* field = kind == INSEM_Q_VELOCITY ? r.sv[dom] : kind == INSEM_Q_ACCELERATION ? r.sa[dom] : r.su[dom];
   for (int c = 0; c < r.ncomp[t]; ++c) {
-    double s = 0.0;
-    for (int32_t q = p0; q < p1; ++q) s += static_cast<double>(r.value[q]) * field[c * nsp + r.pts[q]];
-    row[c] = s;
+    const double s = ordered_dot(r.value, field, r.pts, c * nsp, p0, p1, sw, sx);
+    if (threadIdx.x == 0) row[c] = s;
   }
 }
 
@@ -443,17 +504,19 @@ inline int blocks_for(int64_t n) { return static_cast<int>((n + kTB - 1) / kTB);
 
 int tail_blocks(int64_t n) { return blocks_for(n); }
 
-hipError_t launch_tail_regular(const DomDev& d, const StepCoef& sc, int corr, int pred, const StepClock& clk,
-                               double* partial, int* n_blocks, hipStream_t s) {
-  const int nb = blocks_for(d.n_reg);
+hipError_t launch_tail_regular(const DomDev& d, int64_t first, const StepCoef& sc, int corr, int pred,
+                               const StepClock& clk, double* partial, int* n_blocks, hipStream_t s) {
+  const int nb = first < d.n_reg ? blocks_for(d.n_reg - first) : 0;
   if (n_blocks) *n_blocks = nb;
   if (nb == 0) return hipSuccess;
   const bool w = d.t_w != nullptr;
 #define INSEM_TAIL(C)                                                                                         \
   if (w)                                                                                                      \
-    hipLaunchKernelGGL((k_tail_regular<C, true>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, clk, partial); \
+    hipLaunchKernelGGL((k_tail_regular<C, true>), dim3(nb), dim3(kTB), 0, s, d, first, sc, corr, pred, clk,    \
+                       partial);                                                                              \
   else                                                                                                        \
-    hipLaunchKernelGGL((k_tail_regular<C, false>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, clk, partial);
+    hipLaunchKernelGGL((k_tail_regular<C, false>), dim3(nb), dim3(kTB), 0, s, d, first, sc, corr, pred, clk,   \
+                       partial);
   switch (d.ncomp) {
     case 1: INSEM_TAIL(1) break;
     case 2: INSEM_TAIL(2) break;
@@ -461,6 +524,20 @@ hipError_t launch_tail_regular(const DomDev& d, const StepCoef& sc, int corr, in
     default: return hipErrorInvalidValue;
   }
 #undef INSEM_TAIL
+  return hipGetLastError();
+}
+
+hipError_t launch_tail_grouped(const DomDev& d, const StepCoef& sc, int corr, int pred, const StepClock& clk,
+                               double* partial, int* n_blocks, hipStream_t s) {
+  const int nb = blocks_for(d.grp.end - d.grp.begin);
+  if (n_blocks) *n_blocks = nb;
+  if (nb == 0) return hipSuccess;
+  switch (d.ncomp) {
+    case 1: hipLaunchKernelGGL((k_tail_grouped<1>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, clk, partial); break;
+    case 2: hipLaunchKernelGGL((k_tail_grouped<2>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, clk, partial); break;
+    case 3: hipLaunchKernelGGL((k_tail_grouped<3>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, clk, partial); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -518,7 +595,7 @@ hipError_t launch_reduce(const double* partial, int n, int width, double* out, i
 
 hipError_t launch_receivers(const ReceiverDev& r, const StepClock& clk, int pv_mode, double half_dt, hipStream_t s) {
   if (r.n_tasks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_receivers, dim3((r.n_tasks + 63) / 64), dim3(64), 0, s, r, clk, pv_mode, half_dt);
+  hipLaunchKernelGGL(k_receivers, dim3(r.n_tasks), dim3(kRecWG), 0, s, r, clk, pv_mode, half_dt);
   return hipGetLastError();
 }
 

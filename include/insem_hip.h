@@ -261,6 +261,14 @@ int insem_hip_kernel_time(insem_hip_ctx* ctx, double* force_ms, int32_t* force_l
 /* Milliseconds and launch count of the element-force launches of one block
  * accumulated by the last insem_hip_kernel_time call. */
 int insem_hip_block_time(insem_hip_ctx* ctx, int32_t domain, int32_t block, double* ms, int32_t* launches);
+/* Point counts of one block's force launch in a time step: owned_slots =
+ * element slots whose point the launch updates itself (points of one
+ * unit-weight slot with no boundary, coupling, source or receiver work; 0
+ * when that fusion is off), points_read = distinct global points the launch
+ * gathers.  Used to count the launch's algorithmic bytes; no counterpart in
+ * the reference. */
+int insem_hip_block_points(insem_hip_ctx* ctx, int32_t domain, int32_t block, int64_t* owned_slots,
+                           int64_t* points_read);
 
 /* Device bytes held by the context. */
 int insem_hip_footprint(insem_hip_ctx* ctx, int64_t* bytes);
