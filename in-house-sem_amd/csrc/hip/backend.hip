@@ -737,7 +737,7 @@ struct FuseStep {
 // Element forces of every block of R from the field u; with `fs`, the owned
 // points are updated in the same launches (partial rows advance by each
 // launch's work-groups; returns the rows used).
-int run_forces(insem_hip_ctx* c, DomainRT& R, const float* u, const FuseStep* fs = nullptr) {
+int run_forces(insem_hip_ctx* c, DomainRT& R, const float* u, hipStream_t stream, const FuseStep* fs = nullptr) {
   const int di = static_cast<int>(&R - c->doms.data());
   int rows = 0;
   for (size_t bi = 0; bi < R.blocks.size(); ++bi) {
@@ -759,7 +759,7 @@ int run_forces(insem_hip_ctx* c, DomainRT& R, const float* u, const FuseStep* fs
     }
     hipEvent_t e0 = nullptr;
     timed_begin(c, &e0);
-    HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, true, a, c->stream));
+    HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, true, a, stream));
     timed_end(c, e0, true);
   }
   return rows;
@@ -787,17 +787,18 @@ StepClock clock_at(const insem_hip_ctx* c, int offset) {
 
 // One compute_acceleration + corrector (+ predictor) over both domains, then
 // the receivers, at step *d_clock + offset.  Energy partials are written on
-// energy steps only (the kernels read the step on the device).
+// energy steps only (the kernels read the step on the device).  The solid
+// goes first: the fluid's coupling term reads the solid's new acceleration
+// (solver.cpp:693-727).
 void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, int pv_mode) {
   const StepCoef sc = step_coef(c, implicit);
   const StepClock clk = clock_at(c, offset);
-  int row = 0;
+  hipStream_t s0 = c->stream;
+  int row = 0, nb = 0;
   for (int medium : {INSEM_MEDIUM_ELASTIC, INSEM_MEDIUM_ACOUSTIC}) {
     const int di = medium == INSEM_MEDIUM_ELASTIC ? c->solid : c->fluid;
     if (di < 0) continue;
     DomainRT& R = c->doms[di];
-    int nb = 0;
-    hipEvent_t e0 = nullptr;
     if (c->fuse) {
       FuseStep fs;
       fs.sc = sc;
@@ -805,24 +806,18 @@ void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, 
       fs.pred = pred;
       fs.clk = clk;
       fs.partial = c->partial + 2 * row;
-      row += run_forces(c, R, R.dd.u, &fs);
-      timed_begin(c, &e0);
-      HIP_TRY(launch_tail_grouped(R.dd, sc, corr, pred, clk, c->partial + 2 * row, &nb, c->stream));
-      row += nb;
-      HIP_TRY(launch_tail_regular(R.dd, R.dd.grp.end, sc, corr, pred, clk, c->partial + 2 * row, &nb, c->stream));
-      timed_end(c, e0, false);
+      row += run_forces(c, R, R.dd.u, s0, &fs);
     } else {
-      run_forces(c, R, R.dd.u);
-      timed_begin(c, &e0);
-      HIP_TRY(launch_tail_regular(R.dd, 0, sc, corr, pred, clk, c->partial + 2 * row, &nb, c->stream));
-      timed_end(c, e0, false);
+      run_forces(c, R, R.dd.u, s0);
     }
-    row += nb;
-    HIP_TRY(launch_tail_special(R.dd, R.sp, sc, corr, pred, clk, c->partial + 2 * row, &nb, c->stream));
+    hipEvent_t e0 = nullptr;
+    timed_begin(c, &e0);
+    HIP_TRY(launch_tail(R.dd, R.sp, sc, corr, pred, clk, c->fuse != 0, c->partial + 2 * row, &nb, s0));
+    timed_end(c, e0, false);
     row += nb;
   }
-  if (c->energy_every > 0) HIP_TRY(launch_energy_row(c->partial, row, clk, c->energy_rows, c->stream));
-  HIP_TRY(launch_receivers(c->rec, clk, pv_mode, 0.5 * c->dt, c->stream));
+  if (c->energy_every > 0) HIP_TRY(launch_energy_row(c->partial, row, clk, c->energy_rows, s0));
+  HIP_TRY(launch_receivers(c->rec, clk, pv_mode, 0.5 * c->dt, s0));
 }
 
 void initialize(insem_hip_ctx* c) {
@@ -1118,7 +1113,7 @@ int insem_hip_internal_forces(insem_hip_ctx* c, int32_t di, const float* u, floa
     std::vector<float> h;
     permute_in(R, u, h);
     HIP_TRY(hipMemcpyAsync(R.tmp, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    run_forces(c, R, R.tmp);
+    run_forces(c, R, R.tmp, c->stream);
     HIP_TRY(launch_assemble(R.dd, c->stream));
     if (dot) {
       int nb = 0;
@@ -1181,7 +1176,7 @@ int insem_hip_lambda_max(insem_hip_ctx* c, const float* start, double* lambda_ou
       for (int it = 0; it < 100; ++it) {
         HIP_TRY(launch_zero_points(R.tmp, R.ng, R.d_dir, R.n_dir, all, R.ncomp, st));
         HIP_TRY(launch_zero_points(R.tmp, R.ng, R.d_axis, R.n_axis, R.axis_mask, R.ncomp, st));
-        run_forces(c, R, R.tmp);
+        run_forces(c, R, R.tmp, c->stream);
         HIP_TRY(launch_assemble(R.dd, st));
         int nb = 0;
         HIP_TRY(launch_power_stats(R.dd, R.tmp, c->partial, state, &nb, st));
@@ -1226,7 +1221,7 @@ int insem_hip_lambda_max(insem_hip_ctx* c, const float* start, double* lambda_ou
           HIP_TRY(launch_reduce(c->partial, nb, 1, nxy, q, st));
         }
         for (DomainRT* R : order) {
-          run_forces(c, *R, R->dd.u);
+          run_forces(c, *R, R->dd.u, st);
           HIP_TRY(launch_assemble(R->dd, st));
           const DomainRT& O = R == &S ? F : S;
           HIP_TRY(launch_coupling_power(R->dd, R->cpl_pts, R->cpl_other, R->cpl_cx, R->cpl_cz, R->n_cpl, R->cpl_role,

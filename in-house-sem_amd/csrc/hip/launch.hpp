@@ -20,19 +20,15 @@ struct StepCoef {
 };
 
 int tail_blocks(int64_t n);
-// Tail of a step on the regular points [first, n_reg): assembly through the
-// transpose CSR, a = M^-1 f, optional corrector, energy partials
-// ([blocks][2]: u.f, kinetic) and the next step's predictor.
-// The step number and whether it is an energy step come from the clock.
-hipError_t launch_tail_regular(const DomDev& d, int64_t first, const StepCoef& sc, int corrector, int predictor,
-                               const StepClock& clk, double* partial, int* n_blocks, hipStream_t s);
-// Same on the grouped points [grp.begin, grp.end), summing the group layout.
-hipError_t launch_tail_grouped(const DomDev& d, const StepCoef& sc, int corrector, int predictor,
-                               const StepClock& clk, double* partial, int* n_blocks, hipStream_t s);
-// Same on the special points, with sources, coupling, Stacey, Dirichlet and
-// axis conditions; writes the pre-predictor values into the side buffers.
-hipError_t launch_tail_special(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corrector,
-                               int predictor, const StepClock& clk, double* partial, int* n_blocks, hipStream_t s);
+// Tail of a step on one domain, one launch: a = M^-1 f from the assembled
+// forces (grouped points from the group layout when `grouped`, the other
+// regular points through the transpose CSR), sources, coupling, Stacey,
+// Dirichlet and axis conditions at the special points (whose pre-predictor
+// values go to the side buffers), optional corrector, energy partials
+// ([blocks][2]: u.f, kinetic) and the next step's predictor.  The step
+// number and whether it is an energy step come from the clock.
+hipError_t launch_tail(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corrector, int predictor,
+                       const StepClock& clk, bool grouped, double* partial, int* n_blocks, hipStream_t s);
 // rows[step / energy_every] = sums of [n][2] partials, on energy steps only.
 hipError_t launch_energy_row(const double* partial, int n, const StepClock& clk, double* rows, hipStream_t s);
 // *base += n (end of a chunk of n steps).

@@ -92,47 +92,38 @@ __device__ __forceinline__ void write_partials(bool energy, double ep, double ek
   }
 }
 
-// Regular points [first, n_reg) assembled through the transpose CSR.
+// Regular points [first, n_reg) assembled through the transpose CSR; vb is
+// the block's index within this part of the launch.
 template <int C, bool W>
-__global__ __launch_bounds__(kTB) void k_tail_regular(DomDev d, int64_t first, StepCoef sc, int corr, int pred,
-                                                      StepClock clk, double* partial) {
-  __shared__ double lds[kTB];
-  const bool energy = clock_energy(clk, clock_step(clk));
-  const int64_t g = first + static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
-  double ep = 0.0, ek = 0.0;
-  if (g < d.n_reg) {
-    float f[C];
-    gather_sum<C, W>(d, g, f);
-    update_point<C>(d, sc, corr, pred, energy, g, f, ep, ek);
-  }
-  write_partials(energy, ep, ek, lds, partial);
+__device__ __forceinline__ void regular_body(const DomDev& d, int64_t first, const StepCoef& sc, int corr, int pred,
+                                             bool energy, int vb, double& ep, double& ek) {
+  const int64_t g = first + static_cast<int64_t>(vb) * kTB + threadIdx.x;
+  if (g >= d.n_reg) return;
+  float f[C];
+  gather_sum<C, W>(d, g, f);
+  update_point<C>(d, sc, corr, pred, energy, g, f, ep, ek);
 }
 
 // Grouped points: the contributions of a point lie in rank order at stride
 // size[q] in the group layout; the sum starts from zero in slot order as the
 // assembly's does.
 template <int C>
-__global__ __launch_bounds__(kTB) void k_tail_grouped(DomDev d, StepCoef sc, int corr, int pred, StepClock clk,
-                                                      double* partial) {
-  __shared__ double lds[kTB];
-  const bool energy = clock_energy(clk, clock_step(clk));
-  const int64_t g = d.grp.begin + static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
-  double ep = 0.0, ek = 0.0;
-  if (g < d.grp.end) {
-    int64_t off, n;
-    int cnt;
-    group_locate(d.grp, g, &off, &n, &cnt);
-    const float* src = d.contrib + off;
-    float f[C];
+__device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc, int corr, int pred, bool energy,
+                                             int vb, double& ep, double& ek) {
+  const int64_t g = d.grp.begin + static_cast<int64_t>(vb) * kTB + threadIdx.x;
+  if (g >= d.grp.end) return;
+  int64_t off, n;
+  int cnt;
+  group_locate(d.grp, g, &off, &n, &cnt);
+  const float* src = d.contrib + off;
+  float f[C];
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      float s = 0.0f;
-      for (int r = 0; r < cnt; ++r) s += src[(c * cnt + r) * n];
-      f[c] = s;
-    }
-    update_point<C>(d, sc, corr, pred, energy, g, f, ep, ek);
+  for (int c = 0; c < C; ++c) {
+    float s = 0.0f;
+    for (int r = 0; r < cnt; ++r) s += src[(c * cnt + r) * n];
+    f[c] = s;
   }
-  write_partials(energy, ep, ek, lds, partial);
+  update_point<C>(d, sc, corr, pred, energy, g, f, ep, ek);
 }
 
 // Small SPD solve without pivoting, in double as the reference's
@@ -157,14 +148,12 @@ __device__ __forceinline__ void solve_small(double* A, double* b) {
   }
 }
 
+// Special points: sources, coupling, Stacey, Dirichlet and axis conditions;
+// the pre-predictor values go to the side buffers.
 template <int C, bool W>
-__global__ __launch_bounds__(kTB) void k_tail_special(DomDev d, SpecialDev sp, StepCoef sc, int corr, int pred,
-                                                      StepClock clk, double* partial) {
-  __shared__ double lds[kTB];
-  const int step = clock_step(clk);
-  const bool energy = clock_energy(clk, step);
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
-  double ep = 0.0, ek = 0.0;
+__device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
+                                             int pred, int step, bool energy, int vb, double& ep, double& ek) {
+  const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
   if (i < sp.n_sp) {
     const int64_t g = d.n_reg + i;
     float f[C];
@@ -251,14 +240,27 @@ __global__ __launch_bounds__(kTB) void k_tail_special(DomDev d, SpecialDev sp, S
       d.a[o] = a[c];
     }
   }
-  if (energy) {
-    const double s0 = block_sum(ep, lds);
-    const double s1 = block_sum(ek, lds);
-    if (threadIdx.x == 0) {
-      partial[2 * blockIdx.x] = s0;
-      partial[2 * blockIdx.x + 1] = s1;
-    }
-  }
+}
+
+// The tail of a step on one domain, one launch: blocks [0, n_sp_blocks) take
+// the special points (first, so that their dependent loads start early),
+// the next n_grp_blocks the grouped points, the rest the regular points
+// [reg_first, n_reg) through the CSR.  Energy partials: [block][2].
+template <int C, bool W>
+__global__ __launch_bounds__(kTB) void k_tail(DomDev d, SpecialDev sp, StepCoef sc, int corr, int pred, StepClock clk,
+                                              double* partial, int n_sp_blocks, int n_grp_blocks, int64_t reg_first) {
+  __shared__ double lds[kTB];
+  const int step = clock_step(clk);
+  const bool energy = clock_energy(clk, step);
+  double ep = 0.0, ek = 0.0;
+  const int b = blockIdx.x;
+  if (b < n_sp_blocks)
+    special_body<C, W>(d, sp, sc, corr, pred, step, energy, b, ep, ek);
+  else if (b < n_sp_blocks + n_grp_blocks)
+    grouped_body<C>(d, sc, corr, pred, energy, b - n_sp_blocks, ep, ek);
+  else
+    regular_body<C, W>(d, reg_first, sc, corr, pred, energy, b - n_sp_blocks - n_grp_blocks, ep, ek);
+  write_partials(energy, ep, ek, lds, partial);
 }
 
 __global__ __launch_bounds__(kTB) void k_predictor(float* u, float* v, const float* a, int64_t n, float rdt,
@@ -504,54 +506,23 @@ inline int blocks_for(int64_t n) { return static_cast<int>((n + kTB - 1) / kTB);
 
 int tail_blocks(int64_t n) { return blocks_for(n); }
 
-hipError_t launch_tail_regular(const DomDev& d, int64_t first, const StepCoef& sc, int corr, int pred,
-                               const StepClock& clk, double* partial, int* n_blocks, hipStream_t s) {
-  const int nb = first < d.n_reg ? blocks_for(d.n_reg - first) : 0;
+hipError_t launch_tail(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr, int pred,
+                       const StepClock& clk, bool grouped, double* partial, int* n_blocks, hipStream_t s) {
+  const int nsp = blocks_for(sp.n_sp);
+  const int ngr = grouped ? blocks_for(d.grp.end - d.grp.begin) : 0;
+  const int64_t first = grouped ? d.grp.end : 0;
+  const int nrg = first < d.n_reg ? blocks_for(d.n_reg - first) : 0;
+  const int nb = nsp + ngr + nrg;
   if (n_blocks) *n_blocks = nb;
   if (nb == 0) return hipSuccess;
   const bool w = d.t_w != nullptr;
-#define INSEM_TAIL(C)                                                                                         \
-  if (w)                                                                                                      \
-    hipLaunchKernelGGL((k_tail_regular<C, true>), dim3(nb), dim3(kTB), 0, s, d, first, sc, corr, pred, clk,    \
-                       partial);                                                                              \
-  else                                                                                                        \
-    hipLaunchKernelGGL((k_tail_regular<C, false>), dim3(nb), dim3(kTB), 0, s, d, first, sc, corr, pred, clk,   \
-                       partial);
-  switch (d.ncomp) {
-    case 1: INSEM_TAIL(1) break;
-    case 2: INSEM_TAIL(2) break;
-    case 3: INSEM_TAIL(3) break;
-    default: return hipErrorInvalidValue;
-  }
-#undef INSEM_TAIL
-  return hipGetLastError();
-}
-
-hipError_t launch_tail_grouped(const DomDev& d, const StepCoef& sc, int corr, int pred, const StepClock& clk,
-                               double* partial, int* n_blocks, hipStream_t s) {
-  const int nb = blocks_for(d.grp.end - d.grp.begin);
-  if (n_blocks) *n_blocks = nb;
-  if (nb == 0) return hipSuccess;
-  switch (d.ncomp) {
-    case 1: hipLaunchKernelGGL((k_tail_grouped<1>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, clk, partial); break;
-    case 2: hipLaunchKernelGGL((k_tail_grouped<2>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, clk, partial); break;
-    case 3: hipLaunchKernelGGL((k_tail_grouped<3>), dim3(nb), dim3(kTB), 0, s, d, sc, corr, pred, clk, partial); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_tail_special(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr, int pred,
-                               const StepClock& clk, double* partial, int* n_blocks, hipStream_t s) {
-  const int nb = blocks_for(sp.n_sp);
-  if (n_blocks) *n_blocks = nb;
-  if (nb == 0) return hipSuccess;
-  const bool w = d.t_w != nullptr;
-#define INSEM_TAIL(C)                                                                                              \
-  if (w)                                                                                                           \
-    hipLaunchKernelGGL((k_tail_special<C, true>), dim3(nb), dim3(kTB), 0, s, d, sp, sc, corr, pred, clk, partial); \
-  else                                                                                                             \
-    hipLaunchKernelGGL((k_tail_special<C, false>), dim3(nb), dim3(kTB), 0, s, d, sp, sc, corr, pred, clk, partial);
+#define INSEM_TAIL(C)                                                                                          \
+  if (w)                                                                                                       \
+    hipLaunchKernelGGL((k_tail<C, true>), dim3(nb), dim3(kTB), 0, s, d, sp, sc, corr, pred, clk, partial, nsp, \
+                       ngr, first);                                                                            \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_tail<C, false>), dim3(nb), dim3(kTB), 0, s, d, sp, sc, corr, pred, clk, partial,     \
+                       nsp, ngr, first);
   switch (d.ncomp) {
     case 1: INSEM_TAIL(1) break;
     case 2: INSEM_TAIL(2) break;

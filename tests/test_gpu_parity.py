@@ -253,8 +253,8 @@ def test_advance_in_chunks_equals_one_call():
             assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("model", ["layered", "borehole_m1"])
-def test_fused_update_is_bitwise_the_tail(model, monkeypatch):
+@pytest.mark.parametrize("model,switch", [("layered", "INSEM_HIP_FUSE"), ("borehole_m1", "INSEM_HIP_FUSE")])
+def test_step_schedules_are_bitwise_equal(model, switch, monkeypatch):
     # Points owned by one element slot are updated inside the force launch;
     # the same run with every point updated by the tail kernels
     # (INSEM_HIP_FUSE=0) must give the same bits in traces and fields.
@@ -266,10 +266,10 @@ def test_fused_update_is_bitwise_the_tail(model, monkeypatch):
 
     fused = make()
     a = fused.solver
-    monkeypatch.setenv("INSEM_HIP_FUSE", "0")
+    monkeypatch.setenv(switch, "0")
     plain = make()
     b = plain.solver
-    monkeypatch.delenv("INSEM_HIP_FUSE")
+    monkeypatch.delenv(switch)
     owned = ctypes.c_int64()
     read = ctypes.c_int64()
     _lib.check_hip(_lib.hip().insem_hip_block_points(ctypes.c_void_p(a.context), 0, 0, ctypes.byref(owned),
