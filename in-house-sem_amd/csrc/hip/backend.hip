@@ -727,42 +727,37 @@ void timed_end(insem_hip_ctx* c, hipEvent_t b, bool force) {
 }
 
 // Update of the owned points fused into a time step's force launches.
-struct FuseStep {
-  StepCoef sc;
-  int corr = 0, pred = 0;
-  StepClock clk;
-  double* partial = nullptr;  // first energy row of this domain's launches
-};
+// Parameters of the force launches of a time step (owned points updated,
+// grouped forces written in the group layout).
+StepParams fused_params(const StepCoef& sc, int corr, int pred, const StepClock& clk, double* partial) {
+  StepParams p;
+  p.fuse = 1;
+  p.corr = corr;
+  p.pred = pred;
+  p.rdt = sc.rdt;
+  p.hdt = sc.hdt;
+  p.hdt2 = sc.hdt2;
+  p.cc = sc.cc;
+  p.clk = clk;
+  p.partial = partial;
+  return p;
+}
 
-// Element forces of every block of R from the field u; with `fs`, the owned
-// points are updated in the same launches (partial rows advance by each
-// launch's work-groups; returns the rows used).
-int run_forces(insem_hip_ctx* c, DomainRT& R, const float* u, hipStream_t stream, const FuseStep* fs = nullptr) {
+// Element forces of every block of R from the field u, one launch per
+// block, f_local for every slot.
+void run_forces(insem_hip_ctx* c, DomainRT& R, const float* u, hipStream_t stream) {
   const int di = static_cast<int>(&R - c->doms.data());
-  int rows = 0;
+  const StepParams p;
   for (size_t bi = 0; bi < R.blocks.size(); ++bi) {
     BlockRT& br = R.blocks[bi];
     c->cur_tag = di * 64 + static_cast<int>(bi);
     ForceArgs a = br.args;
     a.u = u;
-    if (fs) {
-      a.fuse = 1;
-      a.rdt = fs->sc.rdt;
-      a.hdt = fs->sc.hdt;
-      a.hdt2 = fs->sc.hdt2;
-      a.cc = fs->sc.cc;
-      a.corr = fs->corr;
-      a.pred = fs->pred;
-      a.clk = fs->clk;
-      a.partial = fs->partial + 2 * rows;
-      rows += force_blocks(c->ngll, br.nb);
-    }
     hipEvent_t e0 = nullptr;
     timed_begin(c, &e0);
-    HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, true, a, stream));
+    HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, true, a, p, stream));
     timed_end(c, e0, true);
   }
-  return rows;
 }
 
 StepCoef step_coef(const insem_hip_ctx* c, bool implicit) {
@@ -785,31 +780,53 @@ StepClock clock_at(const insem_hip_ctx* c, int offset) {
   return k;
 }
 
-// One compute_acceleration + corrector (+ predictor) over both domains, then
-// the receivers, at step *d_clock + offset.  Energy partials are written on
-// energy steps only (the kernels read the step on the device).  The solid
-// goes first: the fluid's coupling term reads the solid's new acceleration
-// (solver.cpp:693-727).
-void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, int pv_mode) {
+// One compute_acceleration + corrector (+ predictor) over both domains at
+// step *d_clock + offset.  Energy partials are written on energy steps only
+// (the kernels read the step on the device).
+//
+// In a time step the element forces of both domains go first (owned points
+// updated in place), then the tail of the solid and that of the fluid: the
+// fluid's coupling term reads the solid's new acceleration
+// (solver.cpp:693-727).  Receivers: rec_prev samples the previous step in
+// the first force launch's first work-groups (before this step's tails
+// rewrite the side buffers they read), rec_now samples this step after its
+// tails.
+void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, int pv_mode, bool rec_prev,
+               bool rec_now) {
   const StepCoef sc = step_coef(c, implicit);
   const StepClock clk = clock_at(c, offset);
   hipStream_t s0 = c->stream;
   int row = 0, nb = 0;
+  bool rec_taken = !rec_prev || c->rec.n_tasks == 0;
   for (int medium : {INSEM_MEDIUM_ELASTIC, INSEM_MEDIUM_ACOUSTIC}) {
     const int di = medium == INSEM_MEDIUM_ELASTIC ? c->solid : c->fluid;
     if (di < 0) continue;
     DomainRT& R = c->doms[di];
-    if (c->fuse) {
-      FuseStep fs;
-      fs.sc = sc;
-      fs.corr = corr;
-      fs.pred = pred;
-      fs.clk = clk;
-      fs.partial = c->partial + 2 * row;
-      row += run_forces(c, R, R.dd.u, s0, &fs);
-    } else {
-      run_forces(c, R, R.dd.u, s0);
+    for (size_t bi = 0; bi < R.blocks.size(); ++bi) {
+      BlockRT& br = R.blocks[bi];
+      StepParams p;
+      if (c->fuse) p = fused_params(sc, corr, pred, clk, c->partial + 2 * row);
+      const bool with_rec = c->fuse && !rec_taken;
+      c->cur_tag = di * 64 + static_cast<int>(bi);
+      hipEvent_t e0 = nullptr;
+      timed_begin(c, &e0);
+      int grid = 0;
+      HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, true, br.args, p, s0, &grid, with_rec ? &c->rec : nullptr,
+                            with_rec ? c->rec.n_tasks : 0, 0.5 * c->dt));
+      timed_end(c, e0, true);
+      rec_taken = rec_taken || with_rec;
+      if (c->fuse) row += grid;
     }
+  }
+  if (!rec_taken) {
+    StepClock prev = clk;
+    prev.offset -= 1;
+    HIP_TRY(launch_receivers(c->rec, prev, 1, 0.5 * c->dt, s0));
+  }
+  for (int medium : {INSEM_MEDIUM_ELASTIC, INSEM_MEDIUM_ACOUSTIC}) {
+    const int di = medium == INSEM_MEDIUM_ELASTIC ? c->solid : c->fluid;
+    if (di < 0) continue;
+    DomainRT& R = c->doms[di];
     hipEvent_t e0 = nullptr;
     timed_begin(c, &e0);
     HIP_TRY(launch_tail(R.dd, R.sp, sc, corr, pred, clk, c->fuse != 0, c->partial + 2 * row, &nb, s0));
@@ -817,11 +834,11 @@ void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, 
     row += nb;
   }
   if (c->energy_every > 0) HIP_TRY(launch_energy_row(c->partial, row, clk, c->energy_rows, s0));
-  HIP_TRY(launch_receivers(c->rec, clk, pv_mode, 0.5 * c->dt, s0));
+  if (rec_now) HIP_TRY(launch_receivers(c->rec, clk, pv_mode, 0.5 * c->dt, s0));
 }
 
 void initialize(insem_hip_ctx* c) {
-  step_body(c, 0, false, 0, 0, 0);
+  step_body(c, 0, false, 0, 0, 0, false, true);
   c->initialized = 1;
 }
 
@@ -832,10 +849,13 @@ void predictor_all(insem_hip_ctx* c) {
 
 // The launches of k steps after *d_clock: a standalone predictor when the
 // chunk opens an advance call, the predictor of each next step fused into
-// the previous tail, none after the last step of the call.
+// the previous tail, none after the last step of the call.  The receivers
+// of each step but the chunk's last ride on the next step's force launch.
 void chunk_launches(insem_hip_ctx* c, int k, bool first, bool last) {
   if (first) predictor_all(c);
-  for (int j = 1; j <= k; ++j) step_body(c, j, true, 1, (last && j == k) ? 0 : 1, 1);
+  const bool carry = c->fuse && c->rec.n_tasks > 0;
+  for (int j = 1; j <= k; ++j)
+    step_body(c, j, true, 1, (last && j == k) ? 0 : 1, 1, carry && j > 1, !carry || j == k);
   HIP_TRY(launch_clock_add(c->d_clock, k, c->stream));
 }
 
@@ -946,6 +966,7 @@ int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) {
       rows += tail_blocks(R.ng) + tail_blocks(R.n_sp) + 2;  // grouped + other regular + special
       for (const BlockRT& br : R.blocks) rows += force_blocks(c->ngll, br.nb);
     }
+    rows += c->rec.n_tasks;  // receiver work-groups of a force launch
     c->partial_rows = rows;
     int dot_rows = 0;
     for (const DomainRT& R : c->doms) dot_rows = std::max(dot_rows, tail_blocks(R.ncomp * R.ng));
@@ -1143,7 +1164,7 @@ int insem_hip_block_forces(insem_hip_ctx* c, int32_t di, int32_t bi, const float
     a.ul = din;
     a.fl = dout;
     a.out_ref = 1;
-    HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, false, a, c->stream));
+    HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, false, a, StepParams(), c->stream));
     HIP_TRY(hipMemcpyAsync(fl, dout, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->allocs.pop_back();

@@ -114,28 +114,32 @@ struct ForceArgs {
   const float* D = nullptr;
   const float* Dj = nullptr;
   const float* w2 = nullptr;
-  // Time step with the update fused in: the points [0, n_int) each belong to
-  // one element slot only (multiplicity 1, unconstrained, no boundary or
-  // source work), so that slot's force is the assembled force and the
-  // work-item finishes the point's update (M^-1, corrector, energy
-  // partials, next predictor) instead of writing f_local.
-  int fuse = 0;
+  // Time step with the update fused in (StepParams::fuse): the points
+  // [0, n_int) each belong to one element slot only (multiplicity 1,
+  // unconstrained, no boundary or source work), so that slot's force is the
+  // assembled force and the work-item finishes the point's update (M^-1,
+  // corrector, energy partials, next predictor) instead of writing f_local.
   int64_t n_int = 0;
   float* uw = nullptr;
   float* v = nullptr;
   float* a = nullptr;
   const float* im = nullptr;
   const float* mass = nullptr;
-  float rdt = 0.0f, hdt = 0.0f, hdt2 = 0.0f;
-  double cc = 0.0;
-  int corr = 0, pred = 0;
-  StepClock clk;
-  double* partial = nullptr;  // [work-group][2]: u.f, kinetic (energy steps)
   // Slots of grouped points write their force into the group layout
   // instead of f_local (fused steps only).
   Groups grp;
   const uint8_t* rank = nullptr;  // [nb][npe]
   float* contrib = nullptr;
+};
+
+// Per-launch parameters of the element-force kernels.
+struct StepParams {
+  int fuse = 0;  // a time step: update owned points, write grouped forces
+  int corr = 0, pred = 0;
+  float rdt = 0.0f, hdt = 0.0f, hdt2 = 0.0f;  // float(dt), float(dt/2), float(dt^2/2)
+  double cc = 0.0;                            // dt^2 / 8 (kinetic energy)
+  StepClock clk;
+  double* partial = nullptr;  // [work-group][2]: u.f, kinetic (energy steps)
 };
 
 // Global-point data of one domain on the device, in the backend's point

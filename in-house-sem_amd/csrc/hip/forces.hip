@@ -2,20 +2,31 @@
 // element block, with the gather from the global field fused in.
 //
 // Mapping: one work-item per element point, 256 work-items per work-group
-// holding floor(256 / npe) whole elements (10 at order 4).  The gathered
-// element fields and the reference fluxes are staged in LDS; each work-item
-// keeps the rows and columns of the derivative matrices it needs in
-// registers.  Every per-point arithmetic expression and every summation
+// holding floor(256 / npe) whole elements (10 at order 4), a "batch".  The
+// gathered element fields and the reference fluxes are staged in LDS; each
+// work-item keeps the rows and columns of the derivative matrices it needs
+// in registers.  Every per-point arithmetic expression and every summation
 // follows the reference kernels (cpp/src/cpu/kernels.cpp:19-431) in the same
 // order, and the library is compiled with -ffp-contract=off, so the result
 // is bitwise the reference's float result.
 //
-// Work-group ids are remapped so that each of the 8 XCDs processes one
-// contiguous range of elements (neighbouring elements share global points,
-// which then stay in that XCD's L2).
+// Global loads are issued in the order their results are needed (the shared
+// matrices, the slot's point, the per-point operands, then everything that
+// depends on the point) and without branches, so that each wait covers only
+// what it must.  Work-group ids are remapped so that each of the 8 XCDs
+// processes one contiguous range of elements (neighbouring elements share
+// global points, which then stay in that XCD's L2).
+//
+// In a time step (StepParams::fuse) the slot's force is consumed directly:
+// a point owned by this slot alone is updated in place, a grouped point's
+// slot writes its force into the group layout, and only the remaining
+// slots (special points, constrained slots, other points) write f_local.
+// The step's first force launch carries the previous step's receiver tasks
+// in its first work-groups.
 
 #include "device.hpp"
 #include "launch.hpp"
+#include "receivers.hpp"
 
 namespace insem_hip {
 namespace {
@@ -78,434 +89,608 @@ __device__ __forceinline__ float spread(float f, int i, int j, const float* g, c
   return f;
 }
 
-template <int NG, int PHYS, bool RECT, bool GATHER>
-__global__ __launch_bounds__(kWG) void k_forces(ForceArgs A) {
+template <int PHYS>
+struct Shape {
+  static constexpr int C = phys_ncomp(PHYS);
+  static constexpr bool AXI = PHYS >= kAcousticAxi0;
+  static constexpr int NAX = PHYS == kElasticAxi1 ? 2 : 1;
+};
+
+// LDS tiles of one work-group.
+template <int NG, int PHYS>
+struct Tiles {
+  static constexpr int NPE = NG * NG, EPB = kWG / NPE, C = Shape<PHYS>::C;
+  static constexpr bool AXI = Shape<PHYS>::AXI;
+  static constexpr int NAX = Shape<PHYS>::NAX;
+  float D[NPE], Dj[NPE], W2[NPE];
+  float u[EPB * C * NPE];
+  float f[EPB * 2 * C * NPE];
+  float g[AXI ? EPB * NAX * NG : 1];
+  float xr[AXI ? EPB * NG : 1];
+  float er[AXI ? EPB * NG : 1];
+};
+
+// Per-point operands of an element slot: metric (xi_x, xi_z, eta_x, eta_z,
+// jac; xi_r ... on axisymmetric blocks), coefficients and the axisymmetric
+// factors.
+struct Ops {
+  float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f, q3 = 0.0f, q4 = 0.0f, k0 = 0.0f, k1 = 0.0f;
+  float qw = 0.0f, qh = 0.0f, qr = 0.0f;
+  bool axial = false;
+};
+
+template <int PHYS, bool RECT>
+__device__ __forceinline__ void load_ops(const ForceArgs& A, int b, int k, Ops& o) {
+  constexpr int C = Shape<PHYS>::C;
+  constexpr bool AXI = Shape<PHYS>::AXI;
+  o.q0 = ld(A.m[0], b, k);
+  o.q3 = ld(A.m[3], b, k);
+  if constexpr (!AXI) o.q4 = ld(A.m[4], b, k);
+  if constexpr (!RECT) {
+    o.q1 = ld(A.m[1], b, k);
+    o.q2 = ld(A.m[2], b, k);
+  }
+  o.k0 = ld(A.c0, b, k);
+  if constexpr (C >= 2) o.k1 = ld(A.c1, b, k);
+  if constexpr (AXI) {
+    o.qw = ld(A.wr, b, k);
+    o.qr = ld(A.r, b, k);
+    if constexpr (PHYS != kAcousticAxi0) o.qh = ld(A.hoop, b, k);
+    o.axial = A.axial[b] != 0;
+  }
+}
+
+// Derivative-matrix rows and columns of point (i, j) (GLJ in xi on axial
+// elements).
+template <int NG, int PHYS>
+__device__ __forceinline__ void d_rows(const Tiles<NG, PHYS>& T, bool axial, int i, int j, float* xr, float* yr,
+                                       float* xc, float* yc) {
+  const float* Dx = axial ? T.Dj : T.D;
+#pragma unroll
+  for (int l = 0; l < NG; ++l) {
+    xr[l] = Dx[i * NG + l];
+    yr[l] = T.D[j * NG + l];
+    xc[l] = Dx[l * NG + i];
+    yc[l] = T.D[l * NG + j];
+  }
+}
+
+// Reference fluxes of point k of element le (from the field tile) into the
+// flux tile, and the direct (undifferentiated) terms of the point.
+template <int NG, int PHYS, bool RECT>
+__device__ __forceinline__ void element_flux(Tiles<NG, PHYS>& T, const Ops& o, int le, int k, int i, int j,
+                                             const float* xr, const float* yr, float& direct0, float& direct1) {
+  constexpr int NPE = NG * NG;
+  constexpr int C = Shape<PHYS>::C;
+  constexpr int NAX = Shape<PHYS>::NAX;
+  const float* U = T.u + le * C * NPE;
+  float* F = T.f + le * 2 * C * NPE;
+  const bool axial = o.axial;
+  direct0 = 0.0f;
+  direct1 = 0.0f;
+  if constexpr (PHYS == kElasticCart) {
+    float ux_xi, ux_eta, uz_xi, uz_eta;
+    deriv<NG>(xr, yr, U, i, j, ux_xi, ux_eta);
+    deriv<NG>(xr, yr, U + NPE, i, j, uz_xi, uz_eta);
+    float xi_x, xi_z = 0.0f, eta_x = 0.0f, eta_z, jac;
+    xi_x = o.q0;
+    eta_z = o.q3;
+    jac = o.q4;
+    if constexpr (!RECT) {
+      xi_z = o.q1;
+      eta_x = o.q2;
+    }
+    float ux_x, ux_z, uz_x, uz_z;
+    if constexpr (RECT) {
+      ux_x = ux_xi * xi_x;
+      ux_z = ux_eta * eta_z;
+      uz_x = uz_xi * xi_x;
+      uz_z = uz_eta * eta_z;
+    } else {
+      pgrad(ux_xi, ux_eta, xi_x, xi_z, eta_x, eta_z, ux_x, ux_z);
+      pgrad(uz_xi, uz_eta, xi_x, xi_z, eta_x, eta_z, uz_x, uz_z);
+    }
+    const float lambda = o.k0, mu = o.k1;
+    const float div = ux_x + uz_z;
+    const float sxx = lambda * div + 2.0f * mu * ux_x;
+    const float szz = lambda * div + 2.0f * mu * uz_z;
+    const float sxz = mu * (ux_z + uz_x);
+    const float wj = T.W2[k] * jac;
+    float f1x, f1z, f2x, f2z;
+    if constexpr (RECT) {
+      f1x = wj * sxx * xi_x;
+      f1z = wj * sxz * xi_x;
+      f2x = wj * sxz * eta_z;
+      f2z = wj * szz * eta_z;
+    } else {
+      rflux(wj, sxx, sxz, xi_x, xi_z, eta_x, eta_z, f1x, f2x);
+      rflux(wj, sxz, szz, xi_x, xi_z, eta_x, eta_z, f1z, f2z);
+    }
+    F[0 * NPE + k] = f1x;
+    F[1 * NPE + k] = f2x;
+    F[2 * NPE + k] = f1z;
+    F[3 * NPE + k] = f2z;
+  } else if constexpr (PHYS == kAcousticCart) {
+    float p_xi, p_eta;
+    deriv<NG>(xr, yr, U, i, j, p_xi, p_eta);
+    float xi_x, xi_z = 0.0f, eta_x = 0.0f, eta_z, jac;
+    xi_x = o.q0;
+    eta_z = o.q3;
+    jac = o.q4;
+    if constexpr (!RECT) {
+      xi_z = o.q1;
+      eta_x = o.q2;
+    }
+    const float c = T.W2[k] * jac * o.k0;
+    float f1, f2;
+    if constexpr (RECT) {
+      f1 = c * p_xi * xi_x * xi_x;
+      f2 = c * p_eta * eta_z * eta_z;
+    } else {
+      float p_x, p_z;
+      pgrad(p_xi, p_eta, xi_x, xi_z, eta_x, eta_z, p_x, p_z);
+      rflux(c, p_x, p_z, xi_x, xi_z, eta_x, eta_z, f1, f2);
+    }
+    F[k] = f1;
+    F[NPE + k] = f2;
+  } else {
+    const float xi_r = o.q0, xi_z = o.q1;
+    const float eta_r = o.q2, eta_z = o.q3;
+    if (axial && i == 0) {
+      T.xr[le * NG + j] = xi_r;
+      T.er[le * NG + j] = eta_r;
+    }
+    if constexpr (PHYS == kAcousticAxi0) {
+      float p_xi, p_eta, p_r, p_z;
+      deriv<NG>(xr, yr, U, i, j, p_xi, p_eta);
+      pgrad(p_xi, p_eta, xi_r, xi_z, eta_r, eta_z, p_r, p_z);
+      float f1, f2;
+      rflux(o.qw * o.k0, p_r, p_z, xi_r, xi_z, eta_r, eta_z, f1, f2);
+      F[k] = f1;
+      F[NPE + k] = f2;
+    } else if constexpr (PHYS == kAcousticAxi1) {
+      float p_xi, p_eta, p_r, p_z;
+      deriv<NG>(xr, yr, U, i, j, p_xi, p_eta);
+      pgrad(p_xi, p_eta, xi_r, xi_z, eta_r, eta_z, p_r, p_z);
+      const float inv_rho = o.k0;
+      const float w = o.qw;
+      float f1, f2;
+      rflux(w * inv_rho, p_r, p_z, xi_r, xi_z, eta_r, eta_z, f1, f2);
+      F[k] = f1;
+      F[NPE + k] = f2;
+      if (axial && i == 0) {
+        direct0 = 0.0f;
+        T.g[le * NG + j] = w * inv_rho * p_r;
+      } else {
+        direct0 = o.qh * inv_rho * U[k] / o.qr;
+      }
+    } else if constexpr (PHYS == kElasticAxi0) {
+      float ur_xi, ur_eta, uz_xi, uz_eta;
+      deriv<NG>(xr, yr, U, i, j, ur_xi, ur_eta);
+      deriv<NG>(xr, yr, U + NPE, i, j, uz_xi, uz_eta);
+      float ur_r, ur_z, uz_r, uz_z;
+      pgrad(ur_xi, ur_eta, xi_r, xi_z, eta_r, eta_z, ur_r, ur_z);
+      pgrad(uz_xi, uz_eta, xi_r, xi_z, eta_r, eta_z, uz_r, uz_z);
+      const float r = o.qr;
+      const bool on_axis = r == 0.0f;
+      const float e_tt = on_axis ? ur_r : U[k] / r;
+      const float lambda = o.k0, mu = o.k1;
+      const float tr = ur_r + uz_z + e_tt;
+      const float srr = lambda * tr + 2 * mu * ur_r;
+      const float szz = lambda * tr + 2 * mu * uz_z;
+      const float stt = lambda * tr + 2 * mu * e_tt;
+      const float srz = mu * (ur_z + uz_r);
+      const float w = o.qw;
+      float f1r, f2r, f1z, f2z;
+      rflux(w, srr, srz, xi_r, xi_z, eta_r, eta_z, f1r, f2r);
+      rflux(w, srz, szz, xi_r, xi_z, eta_r, eta_z, f1z, f2z);
+      F[0 * NPE + k] = f1r;
+      F[1 * NPE + k] = f2r;
+      F[2 * NPE + k] = f1z;
+      F[3 * NPE + k] = f2z;
+      if (axial && i == 0) {
+        direct0 = 0.0f;
+        T.g[le * NG + j] = w * stt;
+      } else {
+        direct0 = o.qh * stt;
+      }
+    } else {  // kElasticAxi1
+      const float s2 = 1.41421356237309504880f;
+      const float is2 = 0.70710678118654752440f;
+      float up_xi, up_eta, um_xi, um_eta, uz_xi, uz_eta;
+      deriv<NG>(xr, yr, U, i, j, up_xi, up_eta);
+      deriv<NG>(xr, yr, U + NPE, i, j, um_xi, um_eta);
+      deriv<NG>(xr, yr, U + 2 * NPE, i, j, uz_xi, uz_eta);
+      float up_r, up_z, um_r, um_z, uz_r, uz_z;
+      pgrad(up_xi, up_eta, xi_r, xi_z, eta_r, eta_z, up_r, up_z);
+      pgrad(um_xi, um_eta, xi_r, xi_z, eta_r, eta_z, um_r, um_z);
+      pgrad(uz_xi, uz_eta, xi_r, xi_z, eta_r, eta_z, uz_r, uz_z);
+      const float r = o.qr;
+      const bool on_axis = r == 0.0f;
+      const float up_over_r = on_axis ? up_r : U[k] / r;
+      const float uz_over_r = on_axis ? uz_r : U[2 * NPE + k] / r;
+      const float ur_r = (up_r + um_r) * is2, ur_z = (up_z + um_z) * is2;
+      const float ut_r = (up_r - um_r) * is2, ut_z = (up_z - um_z) * is2;
+      const float e_tt = s2 * up_over_r;
+      const float lambda = o.k0, mu = o.k1;
+      const float tr = ur_r + uz_z + e_tt;
+      const float srr = lambda * tr + 2 * mu * ur_r;
+      const float szz = lambda * tr + 2 * mu * uz_z;
+      const float stt = lambda * tr + 2 * mu * e_tt;
+      const float srz = mu * (ur_z + uz_r);
+      const float srt = mu * (ut_r - s2 * up_over_r);
+      const float stz = mu * (ut_z - uz_over_r);
+      const float w = o.qw;
+      float a0, a1;
+      rflux(w, (srr + srt) * is2, (srz + stz) * is2, xi_r, xi_z, eta_r, eta_z, a0, a1);
+      F[0 * NPE + k] = a0;
+      F[1 * NPE + k] = a1;
+      rflux(w, (srr - srt) * is2, (srz - stz) * is2, xi_r, xi_z, eta_r, eta_z, a0, a1);
+      F[2 * NPE + k] = a0;
+      F[3 * NPE + k] = a1;
+      rflux(w, srz, szz, xi_r, xi_z, eta_r, eta_z, a0, a1);
+      F[4 * NPE + k] = a0;
+      F[5 * NPE + k] = a1;
+      const float cp = s2 * (stt - srt), cz = -stz;
+      if (axial && i == 0) {
+        direct0 = direct1 = 0.0f;
+        T.g[(le * NAX + 0) * NG + j] = w * cp;
+        T.g[(le * NAX + 1) * NG + j] = w * cz;
+      } else {
+        const float hp = o.qh;
+        direct0 = hp * cp;
+        direct1 = hp * cz;
+      }
+    }
+  }
+}
+
+// Weak divergence of the flux tile at point k (and the axis spread): the
+// slot's force per component.
+template <int NG, int PHYS>
+__device__ __forceinline__ void element_out(const Tiles<NG, PHYS>& T, bool axial, int le, int k, int i, int j,
+                                            const float* xc, const float* yc, float direct0, float direct1,
+                                            float* fo) {
+  constexpr int NPE = NG * NG;
+  constexpr int C = Shape<PHYS>::C;
+  constexpr int NAX = Shape<PHYS>::NAX;
+  const float* F = T.f + le * 2 * C * NPE;
+  if constexpr (PHYS == kElasticCart) {
+    fo[0] = -wdiv<NG>(xc, yc, F, F + NPE, i, j);
+    fo[1] = -wdiv<NG>(xc, yc, F + 2 * NPE, F + 3 * NPE, i, j);
+  } else if constexpr (PHYS == kAcousticCart || PHYS == kAcousticAxi0) {
+    fo[0] = -wdiv<NG>(xc, yc, F, F + NPE, i, j);
+  } else if constexpr (PHYS == kAcousticAxi1) {
+    float fe = -(wdiv<NG>(xc, yc, F, F + NPE, i, j) + direct0);
+    if (axial) fe = spread<NG>(fe, i, j, T.g + le * NG, T.xr + le * NG, T.er + le * NG, T.Dj, T.D);
+    fo[0] = fe;
+  } else if constexpr (PHYS == kElasticAxi0) {
+    float fr = -(wdiv<NG>(xc, yc, F, F + NPE, i, j) + direct0);
+    if (axial) fr = spread<NG>(fr, i, j, T.g + le * NG, T.xr + le * NG, T.er + le * NG, T.Dj, T.D);
+    fo[0] = fr;
+    fo[1] = -wdiv<NG>(xc, yc, F + 2 * NPE, F + 3 * NPE, i, j);
+  } else {
+    float fp = -(wdiv<NG>(xc, yc, F, F + NPE, i, j) + direct0);
+    const float fm = -wdiv<NG>(xc, yc, F + 2 * NPE, F + 3 * NPE, i, j);
+    float fz = -(wdiv<NG>(xc, yc, F + 4 * NPE, F + 5 * NPE, i, j) + direct1);
+    if (axial) {
+      fp = spread<NG>(fp, i, j, T.g + (le * NAX + 0) * NG, T.xr + le * NG, T.er + le * NG, T.Dj, T.D);
+      fz = spread<NG>(fz, i, j, T.g + (le * NAX + 1) * NG, T.xr + le * NG, T.er + le * NG, T.Dj, T.D);
+    }
+    fo[0] = fp;
+    fo[1] = fm;
+    fo[2] = fz;
+  }
+}
+
+template <int NG, int PHYS>
+__device__ __forceinline__ void stage_matrices(const ForceArgs& A, Tiles<NG, PHYS>& T) {
+  constexpr int NPE = NG * NG;
+  for (int q = threadIdx.x; q < NPE; q += kWG) {
+    T.D[q] = A.D[q];
+    T.Dj[q] = A.Dj[q];
+    T.W2[q] = A.w2[q];
+  }
+}
+
+// ------------------------------------------------------------------ seam
+
+// Operator seam: element-local input in the reference layout, one batch per
+// work-group, forces to f_local at elem[b] (out_ref) or out_pos0 + b.
+template <int NG, int PHYS, bool RECT>
+__global__ __launch_bounds__(kWG) void k_forces_local(ForceArgs A) {
   constexpr int NPE = NG * NG;
   constexpr int EPB = kWG / NPE;
-  constexpr int C = phys_ncomp(PHYS);
-  constexpr bool AXI = PHYS >= kAcousticAxi0;
-  constexpr int NAX = PHYS == kElasticAxi1 ? 2 : 1;
-  __shared__ float sD[NPE], sDj[NPE], sW2[NPE];
-  __shared__ float su[EPB * C * NPE];
-  __shared__ float sf[EPB * 2 * C * NPE];
-  __shared__ float sg[AXI ? EPB * NAX * NG : 1];
-  __shared__ float sxr[AXI ? EPB * NG : 1];
-  __shared__ float ser[AXI ? EPB * NG : 1];
-
+  constexpr int C = Shape<PHYS>::C;
+  __shared__ Tiles<NG, PHYS> T;
   const int t = threadIdx.x;
   const int le = t / NPE;
   const int k = t - le * NPE;
   const int i = k % NG, j = k / NG;
-  const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
-  const int b = bid * EPB + le;
+  const int b = xcd_contiguous(blockIdx.x, gridDim.x) * EPB + le;
   const bool act = le < EPB && b < A.nb;
-
-  // Global loads are issued in the order their results are needed (the
-  // shared matrices, the slot's point, the per-point operands, then
-  // everything that depends on the point), so that each wait covers only
-  // what it must.
-  for (int q = t; q < NPE; q += kWG) {
-    sD[q] = A.D[q];
-    sDj[q] = A.Dj[q];
-    sW2[q] = A.w2[q];
-  }
-  int32_t slot_point = -1;  // global point of this slot (< 0: constrained)
-  float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f, q3 = 0.0f, q4 = 0.0f, k0 = 0.0f, k1 = 0.0f;
-  float qw = 0.0f, qh = 0.0f, qr = 0.0f;
-  int prk = 0;              // fused step: rank of a grouped point's slot
-  bool owned = false;       // fused step: this work-item updates the point
-  float pv[C], pim = 0.0f;  // its v and 1/m, loaded with the gather
-  float* U = su + le * C * NPE;
-  float* F = sf + le * 2 * C * NPE;
+  stage_matrices(A, T);
+  Ops o;
   if (act) {
-    int32_t m = 0;
-    if constexpr (GATHER) m = A.l2g[static_cast<size_t>(b) * NPE + k];
-    q0 = ld(A.m[0], b, k);
-    q3 = ld(A.m[3], b, k);
-    if constexpr (!AXI) q4 = ld(A.m[4], b, k);
-    if constexpr (!RECT) {
-      q1 = ld(A.m[1], b, k);
-      q2 = ld(A.m[2], b, k);
-    }
-    k0 = ld(A.c0, b, k);
-    if constexpr (C >= 2) k1 = ld(A.c1, b, k);
-    if constexpr (AXI) {
-      qw = ld(A.wr, b, k);
-      qr = ld(A.r, b, k);
-      if constexpr (PHYS != kAcousticAxi0) qh = ld(A.hoop, b, k);
-    }
-    float uu[C];
-    if constexpr (GATHER) {
-      prk = A.rank[static_cast<size_t>(b) * NPE + k];
-      slot_point = m;
-      // Branch-free loads (lanes without a use read point 0), so that the
-      // waits for the gathered values do not also wait for later loads.
-      const int64_t mg = m >= 0 ? m : 0;
+    load_ops<PHYS, RECT>(A, b, k, o);
+    const size_t e = static_cast<size_t>(A.elem[b]);
 #pragma unroll
-      for (int c = 0; c < C; ++c) uu[c] = A.u[c * A.ng + mg];
-      owned = A.fuse && m >= 0 && m < A.n_int;
-      const int64_t mo = owned ? m : 0;
-#pragma unroll
-      for (int c = 0; c < C; ++c) pv[c] = A.v[c * A.ng + mo];
-      pim = A.im[mo];
-      if (m < 0) {
-        const int32_t r = -1 - m;
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          float s = 0.0f;
-          for (int32_t q = A.row_off[r]; q < A.row_off[r + 1]; ++q) s += A.row_w[q] * A.u[c * A.ng + A.row_dofs[q]];
-          uu[c] = s;
-        }
-      }
-    } else {
-      const size_t e = static_cast<size_t>(A.elem[b]);
-#pragma unroll
-      for (int c = 0; c < C; ++c) uu[c] = A.ul[(e * C + c) * NPE + k];
-    }
-#pragma unroll
-    for (int c = 0; c < C; ++c) U[c * NPE + k] = uu[c];
+    for (int c = 0; c < C; ++c) T.u[le * C * NPE + c * NPE + k] = A.ul[(e * C + c) * NPE + k];
   }
   __syncthreads();
-
-  // Derivative-matrix rows and columns of this point.
-  bool axial = false;
-  if constexpr (AXI) axial = act && A.axial[b] != 0;
-  const float* Dx = axial ? sDj : sD;
   float xr[NG], yr[NG], xc[NG], yc[NG];
-#pragma unroll
-  for (int l = 0; l < NG; ++l) {
-    xr[l] = Dx[i * NG + l];
-    yr[l] = sD[j * NG + l];
-    xc[l] = Dx[l * NG + i];
-    yc[l] = sD[l * NG + j];
-  }
-
+  d_rows(T, o.axial, i, j, xr, yr, xc, yc);
   float direct0 = 0.0f, direct1 = 0.0f;
-  if (act) {
-    if constexpr (PHYS == kElasticCart) {
-      float ux_xi, ux_eta, uz_xi, uz_eta;
-      deriv<NG>(xr, yr, U, i, j, ux_xi, ux_eta);
-      deriv<NG>(xr, yr, U + NPE, i, j, uz_xi, uz_eta);
-      float xi_x, xi_z = 0.0f, eta_x = 0.0f, eta_z, jac;
-      xi_x = q0;
-      eta_z = q3;
-      jac = q4;
-      if constexpr (!RECT) {
-        xi_z = q1;
-        eta_x = q2;
-      }
-      float ux_x, ux_z, uz_x, uz_z;
-      if constexpr (RECT) {
-        ux_x = ux_xi * xi_x;
-        ux_z = ux_eta * eta_z;
-        uz_x = uz_xi * xi_x;
-        uz_z = uz_eta * eta_z;
-      } else {
-        pgrad(ux_xi, ux_eta, xi_x, xi_z, eta_x, eta_z, ux_x, ux_z);
-        pgrad(uz_xi, uz_eta, xi_x, xi_z, eta_x, eta_z, uz_x, uz_z);
-      }
-      const float lambda = k0, mu = k1;
-      const float div = ux_x + uz_z;
-      const float sxx = lambda * div + 2.0f * mu * ux_x;
-      const float szz = lambda * div + 2.0f * mu * uz_z;
-      const float sxz = mu * (ux_z + uz_x);
-      const float wj = sW2[k] * jac;
-      float f1x, f1z, f2x, f2z;
-      if constexpr (RECT) {
-        f1x = wj * sxx * xi_x;
-        f1z = wj * sxz * xi_x;
-        f2x = wj * sxz * eta_z;
-        f2z = wj * szz * eta_z;
-      } else {
-        rflux(wj, sxx, sxz, xi_x, xi_z, eta_x, eta_z, f1x, f2x);
-        rflux(wj, sxz, szz, xi_x, xi_z, eta_x, eta_z, f1z, f2z);
-      }
-      F[0 * NPE + k] = f1x;
-      F[1 * NPE + k] = f2x;
-      F[2 * NPE + k] = f1z;
-      F[3 * NPE + k] = f2z;
-    } else if constexpr (PHYS == kAcousticCart) {
-      float p_xi, p_eta;
-      deriv<NG>(xr, yr, U, i, j, p_xi, p_eta);
-      float xi_x, xi_z = 0.0f, eta_x = 0.0f, eta_z, jac;
-      xi_x = q0;
-      eta_z = q3;
-      jac = q4;
-      if constexpr (!RECT) {
-        xi_z = q1;
-        eta_x = q2;
-      }
-      const float c = sW2[k] * jac * k0;
-      float f1, f2;
-      if constexpr (RECT) {
-        f1 = c * p_xi * xi_x * xi_x;
-        f2 = c * p_eta * eta_z * eta_z;
-      } else {
-        float p_x, p_z;
-        pgrad(p_xi, p_eta, xi_x, xi_z, eta_x, eta_z, p_x, p_z);
-        rflux(c, p_x, p_z, xi_x, xi_z, eta_x, eta_z, f1, f2);
-      }
-      F[k] = f1;
-      F[NPE + k] = f2;
-    } else {
-      const float xi_r = q0, xi_z = q1;
-      const float eta_r = q2, eta_z = q3;
-      if (axial && i == 0) {
-        sxr[le * NG + j] = xi_r;
-        ser[le * NG + j] = eta_r;
-      }
-      if constexpr (PHYS == kAcousticAxi0) {
-        float p_xi, p_eta, p_r, p_z;
-        deriv<NG>(xr, yr, U, i, j, p_xi, p_eta);
-        pgrad(p_xi, p_eta, xi_r, xi_z, eta_r, eta_z, p_r, p_z);
-        float f1, f2;
-        rflux(qw * k0, p_r, p_z, xi_r, xi_z, eta_r, eta_z, f1, f2);
-        F[k] = f1;
-        F[NPE + k] = f2;
-      } else if constexpr (PHYS == kAcousticAxi1) {
-        float p_xi, p_eta, p_r, p_z;
-        deriv<NG>(xr, yr, U, i, j, p_xi, p_eta);
-        pgrad(p_xi, p_eta, xi_r, xi_z, eta_r, eta_z, p_r, p_z);
-        const float inv_rho = k0;
-        const float w = qw;
-        float f1, f2;
-        rflux(w * inv_rho, p_r, p_z, xi_r, xi_z, eta_r, eta_z, f1, f2);
-        F[k] = f1;
-        F[NPE + k] = f2;
-        if (axial && i == 0) {
-          direct0 = 0.0f;
-          sg[le * NG + j] = w * inv_rho * p_r;
-        } else {
-          direct0 = qh * inv_rho * U[k] / qr;
-        }
-      } else if constexpr (PHYS == kElasticAxi0) {
-        float ur_xi, ur_eta, uz_xi, uz_eta;
-        deriv<NG>(xr, yr, U, i, j, ur_xi, ur_eta);
-        deriv<NG>(xr, yr, U + NPE, i, j, uz_xi, uz_eta);
-        float ur_r, ur_z, uz_r, uz_z;
-        pgrad(ur_xi, ur_eta, xi_r, xi_z, eta_r, eta_z, ur_r, ur_z);
-        pgrad(uz_xi, uz_eta, xi_r, xi_z, eta_r, eta_z, uz_r, uz_z);
-        const float r = qr;
-        const bool on_axis = r == 0.0f;
-        const float e_tt = on_axis ? ur_r : U[k] / r;
-        const float lambda = k0, mu = k1;
-        const float tr = ur_r + uz_z + e_tt;
-        const float srr = lambda * tr + 2 * mu * ur_r;
-        const float szz = lambda * tr + 2 * mu * uz_z;
-        const float stt = lambda * tr + 2 * mu * e_tt;
-        const float srz = mu * (ur_z + uz_r);
-        const float w = qw;
-        float f1r, f2r, f1z, f2z;
-        rflux(w, srr, srz, xi_r, xi_z, eta_r, eta_z, f1r, f2r);
-        rflux(w, srz, szz, xi_r, xi_z, eta_r, eta_z, f1z, f2z);
-        F[0 * NPE + k] = f1r;
-        F[1 * NPE + k] = f2r;
-        F[2 * NPE + k] = f1z;
-        F[3 * NPE + k] = f2z;
-        if (axial && i == 0) {
-          direct0 = 0.0f;
-          sg[le * NG + j] = w * stt;
-        } else {
-          direct0 = qh * stt;
-        }
-      } else {  // kElasticAxi1
-        const float s2 = 1.41421356237309504880f;
-        const float is2 = 0.70710678118654752440f;
-        float up_xi, up_eta, um_xi, um_eta, uz_xi, uz_eta;
-        deriv<NG>(xr, yr, U, i, j, up_xi, up_eta);
-        deriv<NG>(xr, yr, U + NPE, i, j, um_xi, um_eta);
-        deriv<NG>(xr, yr, U + 2 * NPE, i, j, uz_xi, uz_eta);
-        float up_r, up_z, um_r, um_z, uz_r, uz_z;
-        pgrad(up_xi, up_eta, xi_r, xi_z, eta_r, eta_z, up_r, up_z);
-        pgrad(um_xi, um_eta, xi_r, xi_z, eta_r, eta_z, um_r, um_z);
-        pgrad(uz_xi, uz_eta, xi_r, xi_z, eta_r, eta_z, uz_r, uz_z);
-        const float r = qr;
-        const bool on_axis = r == 0.0f;
-        const float up_over_r = on_axis ? up_r : U[k] / r;
-        const float uz_over_r = on_axis ? uz_r : U[2 * NPE + k] / r;
-        const float ur_r = (up_r + um_r) * is2, ur_z = (up_z + um_z) * is2;
-        const float ut_r = (up_r - um_r) * is2, ut_z = (up_z - um_z) * is2;
-        const float e_tt = s2 * up_over_r;
-        const float lambda = k0, mu = k1;
-        const float tr = ur_r + uz_z + e_tt;
-        const float srr = lambda * tr + 2 * mu * ur_r;
-        const float szz = lambda * tr + 2 * mu * uz_z;
-        const float stt = lambda * tr + 2 * mu * e_tt;
-        const float srz = mu * (ur_z + uz_r);
-        const float srt = mu * (ut_r - s2 * up_over_r);
-        const float stz = mu * (ut_z - uz_over_r);
-        const float w = qw;
-        float a0, a1;
-        rflux(w, (srr + srt) * is2, (srz + stz) * is2, xi_r, xi_z, eta_r, eta_z, a0, a1);
-        F[0 * NPE + k] = a0;
-        F[1 * NPE + k] = a1;
-        rflux(w, (srr - srt) * is2, (srz - stz) * is2, xi_r, xi_z, eta_r, eta_z, a0, a1);
-        F[2 * NPE + k] = a0;
-        F[3 * NPE + k] = a1;
-        rflux(w, srz, szz, xi_r, xi_z, eta_r, eta_z, a0, a1);
-        F[4 * NPE + k] = a0;
-        F[5 * NPE + k] = a1;
-        const float cp = s2 * (stt - srt), cz = -stz;
-        if (axial && i == 0) {
-          direct0 = direct1 = 0.0f;
-          sg[(le * NAX + 0) * NG + j] = w * cp;
-          sg[(le * NAX + 1) * NG + j] = w * cz;
-        } else {
-          const float hp = qh;
-          direct0 = hp * cp;
-          direct1 = hp * cz;
-        }
-      }
-    }
-  }
+  if (act) element_flux<NG, PHYS, RECT>(T, o, le, k, i, j, xr, yr, direct0, direct1);
   __syncthreads();
-
+  if (!act) return;
   float fo[C];
-  if (act) {
-    if constexpr (PHYS == kElasticCart) {
-      fo[0] = -wdiv<NG>(xc, yc, F, F + NPE, i, j);
-      fo[1] = -wdiv<NG>(xc, yc, F + 2 * NPE, F + 3 * NPE, i, j);
-    } else if constexpr (PHYS == kAcousticCart || PHYS == kAcousticAxi0) {
-      fo[0] = -wdiv<NG>(xc, yc, F, F + NPE, i, j);
-    } else if constexpr (PHYS == kAcousticAxi1) {
-      float fe = -(wdiv<NG>(xc, yc, F, F + NPE, i, j) + direct0);
-      if (axial) fe = spread<NG>(fe, i, j, sg + le * NG, sxr + le * NG, ser + le * NG, sDj, sD);
-      fo[0] = fe;
-    } else if constexpr (PHYS == kElasticAxi0) {
-      float fr = -(wdiv<NG>(xc, yc, F, F + NPE, i, j) + direct0);
-      if (axial) fr = spread<NG>(fr, i, j, sg + le * NG, sxr + le * NG, ser + le * NG, sDj, sD);
-      fo[0] = fr;
-      fo[1] = -wdiv<NG>(xc, yc, F + 2 * NPE, F + 3 * NPE, i, j);
-    } else {
-      float fp = -(wdiv<NG>(xc, yc, F, F + NPE, i, j) + direct0);
-      const float fm = -wdiv<NG>(xc, yc, F + 2 * NPE, F + 3 * NPE, i, j);
-      float fz = -(wdiv<NG>(xc, yc, F + 4 * NPE, F + 5 * NPE, i, j) + direct1);
-      if (axial) {
-        fp = spread<NG>(fp, i, j, sg + (le * NAX + 0) * NG, sxr + le * NG, ser + le * NG, sDj, sD);
-        fz = spread<NG>(fz, i, j, sg + (le * NAX + 1) * NG, sxr + le * NG, ser + le * NG, sDj, sD);
-      }
-      fo[0] = fp;
-      fo[1] = fm;
-      fo[2] = fz;
-    }
-  }
-
-  bool done = false;
-  if constexpr (GATHER) {
-    if (A.fuse) {
-      const bool energy = clock_energy(A.clk, clock_step(A.clk));
-      double ep = 0.0, ek = 0.0;
-      if (act && owned) {
-        // The slot's force is the point's assembled force (0.0f + f, the
-        // assembly's sum of one term); the update follows the tail's
-        // expressions in the same order (update.hip).
-        done = true;
-        const int64_t g = slot_point;
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const int64_t o = c * A.ng + g;
-          float u = U[c * NPE + k], v = pv[c];
-          const float f = 0.0f + fo[c];
-          const float a = f * pim;
-          if (energy) ep += static_cast<double>(u) * static_cast<double>(f);
-          if (A.corr) v = v + A.hdt * a;
-          if (energy) {
-            const double vd = v, ad = a;
-            ek += static_cast<double>(A.mass[g]) * (0.5 * vd * vd - A.cc * ad * ad);
-          }
-          if (A.pred) {
-            u = u + (A.rdt * v + A.hdt2 * a);
-            v = v + A.hdt * a;
-            A.uw[o] = u;
-          } else {
-            A.a[o] = a;
-          }
-          A.v[o] = v;
-        }
-      } else if (act && slot_point >= A.grp.begin && slot_point < A.grp.end) {
-        done = true;
-        int64_t off, n;
-        int cnt;
-        group_locate(A.grp, slot_point, &off, &n, &cnt);
-        float* dst = A.contrib + off + prk * n;
-#pragma unroll
-        for (int c = 0; c < C; ++c) dst[c * cnt * n] = fo[c];
-      }
-      if (energy) {
-        // Fixed-tree work-group sums (every work-item takes part).
-        __shared__ double red[kWG];
-        double s2[2] = {ep, ek};
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          red[t] = s2[q];
-          __syncthreads();
-          for (int w = kWG / 2; w > 0; w >>= 1) {
-            if (t < w) red[t] += red[t + w];
-            __syncthreads();
-          }
-          if (t == 0) A.partial[2 * blockIdx.x + q] = red[0];
-          __syncthreads();
-        }
-      }
-    }
-  }
-  if (!act || done) return;
-
+  element_out<NG, PHYS>(T, o.axial, le, k, i, j, xc, yc, direct0, direct1, fo);
   const size_t pos = A.out_ref ? static_cast<size_t>(A.elem[b]) : static_cast<size_t>(A.out_pos0) + b;
   float* out = A.fl + pos * C * NPE;
 #pragma unroll
   for (int c = 0; c < C; ++c) out[c * NPE + k] = fo[c];
 }
 
+// ------------------------------------------------------------ production
+
+// Registers of one element slot.
+template <int C>
+struct Slot {
+  bool act = false;
+  int b = 0;
+  int32_t m = -1;  // global point (< 0: constrained slot of row -1 - m)
+  Ops o;
+  int rank = 0;
+  bool owned = false;
+  float uu[C];
+  float pv[C];
+  float pim = 0.0f;
+};
+
+// First round trip: the slot's point, operands and rank.
+template <int NG, int PHYS, bool RECT>
+__device__ __forceinline__ void stage_point(const ForceArgs& A, int batch, int le, int k, Slot<Shape<PHYS>::C>& s) {
+  constexpr int NPE = NG * NG;
+  constexpr int EPB = kWG / NPE;
+  const int b = batch * EPB + le;
+  s.act = le < EPB && b < A.nb;
+  s.b = b;
+  if (s.act) {
+    s.m = A.l2g[static_cast<size_t>(b) * NPE + k];
+    load_ops<PHYS, RECT>(A, b, k, s.o);
+    s.rank = A.rank[static_cast<size_t>(b) * NPE + k];
+  }
+}
+
+// Second round trip: the gathered field value and, for an owned point in a
+// time step, its v and 1/m.  Lanes without a use read point 0 (no branch,
+// so that the waits for these values do not also cover later loads).
+template <int C>
+__device__ __forceinline__ void stage_gather(const ForceArgs& A, int fuse, Slot<C>& s) {
+  if (!s.act) return;
+  const int32_t m = s.m;
+  const int64_t mg = m >= 0 ? m : 0;
+#pragma unroll
+  for (int c = 0; c < C; ++c) s.uu[c] = A.u[c * A.ng + mg];
+  s.owned = fuse && m >= 0 && m < A.n_int;
+  const int64_t mo = s.owned ? m : 0;
+#pragma unroll
+  for (int c = 0; c < C; ++c) s.pv[c] = A.v[c * A.ng + mo];
+  s.pim = A.im[mo];
+  if (m < 0) {
+    const int32_t r = -1 - m;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      float v = 0.0f;
+      for (int32_t q = A.row_off[r]; q < A.row_off[r + 1]; ++q) v += A.row_w[q] * A.u[c * A.ng + A.row_dofs[q]];
+      s.uu[c] = v;
+    }
+  }
+}
+
+// Consumes the slot's force fo.
+template <int C, int NPE>
+__device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepParams& P, const Slot<C>& s, int k,
+                                              const float* U, const float* fo, bool energy, double& ep, double& ek) {
+  if (s.owned) {
+    // The slot's force is the point's assembled force (0.0f + f, the
+    // assembly's sum of one term); the update follows the tail's
+    // expressions in the same order (update.hip).
+    const int64_t g = s.m;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int64_t o = c * A.ng + g;
+      float u = U[c * NPE + k], v = s.pv[c];
+      const float f = 0.0f + fo[c];
+      const float a = f * s.pim;
+      if (energy) ep += static_cast<double>(u) * static_cast<double>(f);
+      if (P.corr) v = v + P.hdt * a;
+      if (energy) {
+        const double vd = v, ad = a;
+        ek += static_cast<double>(A.mass[g]) * (0.5 * vd * vd - P.cc * ad * ad);
+      }
+      if (P.pred) {
+        u = u + (P.rdt * v + P.hdt2 * a);
+        v = v + P.hdt * a;
+        A.uw[o] = u;
+      } else {
+        A.a[o] = a;
+      }
+      A.v[o] = v;
+    }
+    return;
+  }
+  if (P.fuse && s.m >= A.grp.begin && s.m < A.grp.end) {
+    int64_t off, n;
+    int cnt;
+    group_locate(A.grp, s.m, &off, &n, &cnt);
+    float* dst = A.contrib + off + s.rank * n;
+#pragma unroll
+    for (int c = 0; c < C; ++c) dst[c * cnt * n] = fo[c];
+    return;
+  }
+  float* out = A.fl + (static_cast<size_t>(A.out_pos0) + s.b) * C * NPE;
+#pragma unroll
+  for (int c = 0; c < C; ++c) out[c * NPE + k] = fo[c];
+}
+
+// One batch of a block from the global field.
+template <int NG, int PHYS, bool RECT>
+__device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams& P, Tiles<NG, PHYS>& T, int batch,
+                                            bool energy, double& ep, double& ek) {
+  constexpr int NPE = NG * NG;
+  constexpr int C = Shape<PHYS>::C;
+  const int t = threadIdx.x;
+  const int le = t / NPE;
+  const int k = t - le * NPE;
+  const int i = k % NG, j = k / NG;
+  stage_matrices(A, T);
+  Slot<C> s;
+  stage_point<NG, PHYS, RECT>(A, batch, le, k, s);
+  stage_gather<C>(A, P.fuse, s);
+  float* U = T.u + le * C * NPE;
+  if (s.act) {
+#pragma unroll
+    for (int c = 0; c < C; ++c) U[c * NPE + k] = s.uu[c];
+  }
+  __syncthreads();
+  float xr[NG], yr[NG], xc[NG], yc[NG];
+  d_rows(T, s.o.axial, i, j, xr, yr, xc, yc);
+  float direct0 = 0.0f, direct1 = 0.0f;
+  if (s.act) element_flux<NG, PHYS, RECT>(T, s.o, le, k, i, j, xr, yr, direct0, direct1);
+  __syncthreads();
+  if (s.act) {
+    float fo[C];
+    element_out<NG, PHYS>(T, s.o.axial, le, k, i, j, xc, yc, direct0, direct1, fo);
+    slot_epilogue<C, NPE>(A, P, s, k, U, fo, energy, ep, ek);
+  }
+}
+
+// Fixed-tree work-group sums of the energy terms into out[0..1] (every
+// work-item takes part).
+__device__ __forceinline__ void write_energy(double ep, double ek, double* red, double* out) {
+  const int t = threadIdx.x;
+  const double v[2] = {ep, ek};
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    __syncthreads();
+    red[t] = v[q];
+    __syncthreads();
+    for (int w = kWG / 2; w > 0; w >>= 1) {
+      if (t < w) red[t] += red[t + w];
+      __syncthreads();
+    }
+    if (t == 0) out[q] = red[0];
+  }
+}
+
+// One block per launch.  In a time step the first launch also carries the
+// previous step's receiver tasks in its first n_rec work-groups (they read
+// the side buffers before this step's tails rewrite them).
+template <int NG, int PHYS, bool RECT>
+struct ForceLds {
+  static constexpr size_t tiles = sizeof(Tiles<NG, PHYS>);
+  static constexpr size_t rec = kWG * (sizeof(double) + sizeof(float));
+  static constexpr size_t bytes = tiles > rec ? tiles : rec;
+};
+
+template <int NG, int PHYS, bool RECT>
+__global__ __launch_bounds__(kWG) void k_forces(ForceArgs A, StepParams P, ReceiverDev R, int n_rec,
+                                                double half_dt) {
+  __shared__ __align__(16) unsigned char smem[ForceLds<NG, PHYS, RECT>::bytes];
+  __shared__ double red[kWG];
+  const int step = P.fuse ? clock_step(P.clk) : 0;
+  const bool energy = P.fuse && clock_energy(P.clk, step);
+  double ep = 0.0, ek = 0.0;
+  const int wg = blockIdx.x;
+  if (wg < n_rec) {
+    double* sw = reinterpret_cast<double*>(smem);
+    float* sx = reinterpret_cast<float*>(smem + kWG * sizeof(double));
+    receiver_task<kWG>(R, wg, P.clk, step - 1, 1, half_dt, sw, sx);
+  } else {
+    force_batch<NG, PHYS, RECT>(A, P, *reinterpret_cast<Tiles<NG, PHYS>*>(smem),
+                                xcd_contiguous(wg - n_rec, gridDim.x - n_rec), energy, ep, ek);
+  }
+  if (energy) write_energy(ep, ek, red, P.partial + 2 * blockIdx.x);
+}
+
+// Extra work of a force launch: receiver tasks in its first work-groups.
+struct RecWork {
+  const ReceiverDev* r = nullptr;
+  int n = 0;
+  double half_dt = 0.0;
+};
+
 template <int NG, int PHYS, bool RECT, bool GATHER>
-hipError_t launch_one(const ForceArgs& a, hipStream_t s) {
+hipError_t launch_one(const ForceArgs& a, const StepParams& p, const RecWork& rw, hipStream_t s, int* grid_out) {
   constexpr int EPB = kWG / (NG * NG);
-  const int grid = (a.nb + EPB - 1) / EPB;
-  if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_forces<NG, PHYS, RECT, GATHER>), dim3(grid), dim3(kWG), 0, s, a);
+  const int batches = (a.nb + EPB - 1) / EPB;
+  if constexpr (GATHER) {
+    const int n_rec = rw.r ? rw.n : 0;
+    const int grid = batches + n_rec;
+    if (grid_out) *grid_out = grid;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_forces<NG, PHYS, RECT>), dim3(grid), dim3(kWG), 0, s, a, p,
+                       rw.r ? *rw.r : ReceiverDev(), n_rec, rw.half_dt);
+  } else {
+    if (grid_out) *grid_out = batches;
+    if (batches == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_forces_local<NG, PHYS, RECT>), dim3(batches), dim3(kWG), 0, s, a);
+  }
   return hipGetLastError();
 }
 
 template <int NG, bool GATHER>
-hipError_t launch_ng(int phys, bool rect, const ForceArgs& a, hipStream_t s) {
+hipError_t launch_ng(int phys, bool rect, const ForceArgs& a, const StepParams& p, const RecWork& w, hipStream_t s,
+                     int* g) {
   switch (phys) {
     case kAcousticCart:
-      return rect ? launch_one<NG, kAcousticCart, true, GATHER>(a, s) : launch_one<NG, kAcousticCart, false, GATHER>(a, s);
+      return rect ? launch_one<NG, kAcousticCart, true, GATHER>(a, p, w, s, g)
+                  : launch_one<NG, kAcousticCart, false, GATHER>(a, p, w, s, g);
     case kElasticCart:
-      return rect ? launch_one<NG, kElasticCart, true, GATHER>(a, s) : launch_one<NG, kElasticCart, false, GATHER>(a, s);
-    case kAcousticAxi0: return launch_one<NG, kAcousticAxi0, false, GATHER>(a, s);
-    case kElasticAxi0: return launch_one<NG, kElasticAxi0, false, GATHER>(a, s);
-    case kAcousticAxi1: return launch_one<NG, kAcousticAxi1, false, GATHER>(a, s);
-    case kElasticAxi1: return launch_one<NG, kElasticAxi1, false, GATHER>(a, s);
+      return rect ? launch_one<NG, kElasticCart, true, GATHER>(a, p, w, s, g)
+                  : launch_one<NG, kElasticCart, false, GATHER>(a, p, w, s, g);
+    case kAcousticAxi0: return launch_one<NG, kAcousticAxi0, false, GATHER>(a, p, w, s, g);
+    case kElasticAxi0: return launch_one<NG, kElasticAxi0, false, GATHER>(a, p, w, s, g);
+    case kAcousticAxi1: return launch_one<NG, kAcousticAxi1, false, GATHER>(a, p, w, s, g);
+    case kElasticAxi1: return launch_one<NG, kElasticAxi1, false, GATHER>(a, p, w, s, g);
   }
   return hipErrorInvalidValue;
 }
 
 template <bool GATHER>
-hipError_t launch_gather_mode(int ngll, int phys, bool rect, const ForceArgs& a, hipStream_t s) {
+hipError_t launch_gather_mode(int ngll, int phys, bool rect, const ForceArgs& a, const StepParams& p, const RecWork& w,
+                              hipStream_t s, int* g) {
   switch (ngll) {
-    case 2: return launch_ng<2, GATHER>(phys, rect, a, s);
-    case 3: return launch_ng<3, GATHER>(phys, rect, a, s);
-    case 4: return launch_ng<4, GATHER>(phys, rect, a, s);
-    case 5: return launch_ng<5, GATHER>(phys, rect, a, s);
-    case 6: return launch_ng<6, GATHER>(phys, rect, a, s);
-    case 7: return launch_ng<7, GATHER>(phys, rect, a, s);
-    case 8: return launch_ng<8, GATHER>(phys, rect, a, s);
-    case 9: return launch_ng<9, GATHER>(phys, rect, a, s);
+    case 2: return launch_ng<2, GATHER>(phys, rect, a, p, w, s, g);
+    case 3: return launch_ng<3, GATHER>(phys, rect, a, p, w, s, g);
+    case 4: return launch_ng<4, GATHER>(phys, rect, a, p, w, s, g);
+    case 5: return launch_ng<5, GATHER>(phys, rect, a, p, w, s, g);
+    case 6: return launch_ng<6, GATHER>(phys, rect, a, p, w, s, g);
+    case 7: return launch_ng<7, GATHER>(phys, rect, a, p, w, s, g);
+    case 8: return launch_ng<8, GATHER>(phys, rect, a, p, w, s, g);
+    case 9: return launch_ng<9, GATHER>(phys, rect, a, p, w, s, g);
   }
   return hipErrorInvalidValue;
 }
 
 }  // namespace
 
-hipError_t launch_forces(int ngll, int phys, bool rect, bool gather, const ForceArgs& a, hipStream_t s) {
-  if (a.fuse && !gather) return hipErrorInvalidValue;
-  return gather ? launch_gather_mode<true>(ngll, phys, rect, a, s) : launch_gather_mode<false>(ngll, phys, rect, a, s);
-}
-
 int force_blocks(int ngll, int nb) {
   const int epb = kWG / (ngll * ngll);
   return (nb + epb - 1) / epb;
+}
+
+hipError_t launch_forces(int ngll, int phys, bool rect, bool gather, const ForceArgs& a, const StepParams& p,
+                         hipStream_t s, int* grid, const ReceiverDev* rec, int n_rec, double half_dt) {
+  if (p.fuse && !gather) return hipErrorInvalidValue;
+  RecWork w;
+  w.r = rec;
+  w.n = n_rec;
+  w.half_dt = half_dt;
+  return gather ? launch_gather_mode<true>(ngll, phys, rect, a, p, w, s, grid)
+                : launch_gather_mode<false>(ngll, phys, rect, a, p, w, s, grid);
 }
 
 }  // namespace insem_hip

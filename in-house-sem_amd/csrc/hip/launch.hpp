@@ -5,11 +5,42 @@
 
 namespace insem_hip {
 
+// Receiver tasks (one per receiver quantity) on the device: interpolation
+// weights over the receiver's points, which index the side buffers of the
+// special points (the pre-predictor u, v, a of each step).
+struct ReceiverDev {
+  int n_tasks = 0;
+  const int32_t* domain = nullptr;  // per task (receiver quantity): side-buffer slot 0/1
+  const int32_t* kind = nullptr;    // INSEM_Q_*
+  const int32_t* ncomp = nullptr;   // raw device components
+  const int32_t* col = nullptr;     // first column in a trace row
+  const int32_t* pt_off = nullptr;  // [n_tasks + 1]
+  const int32_t* pts = nullptr;     // side-buffer index of each point
+  const float* value = nullptr;
+  const float* d_dx = nullptr;
+  const float* d_dz = nullptr;
+  const double* rho = nullptr;
+  double* pv = nullptr;  // [task][4]: prev_ax, prev_az, vel_x, vel_z
+  int n_cols = 0;
+  double* traces = nullptr;  // [sample][n_cols]
+  const float* su[2] = {nullptr, nullptr};
+  const float* sv[2] = {nullptr, nullptr};
+  const float* sa[2] = {nullptr, nullptr};
+  int64_t n_sp[2] = {0, 0};
+};
+
 // Element forces of one block (gather = read the global field through l2g;
-// otherwise read the element-local field in the reference layout).
-// With a.fuse (gather only), the points [0, a.n_int) are updated in the
-// kernel and their energy partials go to a.partial[work-group][2].
-hipError_t launch_forces(int ngll, int phys, bool rect, bool gather, const ForceArgs& a, hipStream_t s);
+// otherwise read the element-local field in the reference layout).  With
+// p.fuse (gather only), the owned points [0, a.n_int) are updated in the
+// kernel, grouped points receive their forces in the group layout and the
+// energy partials go to p.partial[work-group][2]; *grid (optional) receives
+// the launch's work-group count.
+hipError_t launch_forces(int ngll, int phys, bool rect, bool gather, const ForceArgs& a, const StepParams& p,
+                         hipStream_t s, int* grid = nullptr, const ReceiverDev* rec = nullptr, int n_rec = 0,
+                         double half_dt = 0.0);
+// With rec (gather only), the launch's first n_rec work-groups sample the
+// receivers at the step before the clock's (integrating the particle
+// velocity), ahead of the element work-groups.
 // Work-groups of one force launch over nb elements.
 int force_blocks(int ngll, int nb);
 
@@ -39,27 +70,6 @@ hipError_t launch_predictor(const DomDev& d, const StepCoef& sc, hipStream_t s);
 hipError_t launch_assemble(const DomDev& d, hipStream_t s);
 // Fixed-order sum of n rows of `width` doubles into out (+= when accumulate).
 hipError_t launch_reduce(const double* partial, int n, int width, double* out, int accumulate, hipStream_t s);
-
-struct ReceiverDev {
-  int n_tasks = 0;
-  const int32_t* domain = nullptr;  // per task (receiver quantity): side-buffer slot 0/1
-  const int32_t* kind = nullptr;    // INSEM_Q_*
-  const int32_t* ncomp = nullptr;   // raw device components
-  const int32_t* col = nullptr;     // first column in a trace row
-  const int32_t* pt_off = nullptr;  // [n_tasks + 1]
-  const int32_t* pts = nullptr;     // side-buffer index of each point
-  const float* value = nullptr;
-  const float* d_dx = nullptr;
-  const float* d_dz = nullptr;
-  const double* rho = nullptr;
-  double* pv = nullptr;  // [task][4]: prev_ax, prev_az, vel_x, vel_z
-  int n_cols = 0;
-  double* traces = nullptr;  // [sample][n_cols]
-  const float* su[2] = {nullptr, nullptr};
-  const float* sv[2] = {nullptr, nullptr};
-  const float* sa[2] = {nullptr, nullptr};
-  int64_t n_sp[2] = {0, 0};
-};
 
 // Records the sample of the clock's step (when it is a recording step) and
 // advances the particle velocity (pv_mode 0: take prev from the current

@@ -14,6 +14,7 @@
 #include "../../../include/insem_hip.h"
 #include "device.hpp"
 #include "launch.hpp"
+#include "receivers.hpp"
 
 namespace insem_hip {
 namespace {
@@ -311,73 +312,12 @@ __global__ void k_clock_add(int32_t* base, int n) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *base += n;
 }
 
-// Receivers (solver.cpp:780-811, 945-958): one work-item per receiver
-// quantity, sums over the receiver's points in their order, accumulated in
-// double from the float fields as the reference does; the particle-velocity
-// trapezoid too.  These O(receivers x npe) reductions are the only FP64 of
-// the step besides the energy monitor.
 constexpr int kRecWG = 64;
-
-// Sums w[q] * x[q] over q in [p0, p1) in ascending q, in double: the lanes
-// stage the operands in LDS, lane 0 adds them in order (the reference's
-// serial order, without a chain of dependent global loads).
-__device__ double ordered_dot(const float* w, const float* x, const int32_t* pts, int64_t stride, int32_t p0,
-                              int32_t p1, double* sw, float* sx) {
-  double s = 0.0;
-  for (int32_t base = p0; base < p1; base += kRecWG) {
-    const int32_t q = base + static_cast<int32_t>(threadIdx.x);
-    if (q < p1) {
-      sw[threadIdx.x] = w[q];
-      sx[threadIdx.x] = x[stride + pts[q]];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int32_t n = p1 - base < kRecWG ? p1 - base : kRecWG;
-      for (int32_t i = 0; i < n; ++i) s += sw[i] * static_cast<double>(sx[i]);
-    }
-    __syncthreads();
-  }
-  return s;
-}
 
 __global__ __launch_bounds__(kRecWG) void k_receivers(ReceiverDev r, StepClock clk, int pv_mode, double half_dt) {
   __shared__ double sw[kRecWG];
   __shared__ float sx[kRecWG];
-  const int t = blockIdx.x;
-  const int step = clock_step(clk);
-  const int sample = step % clk.record_every == 0 && step / clk.record_every < clk.max_samples
-                         ? step / clk.record_every
-                         : -1;
-  const int dom = r.domain[t];
-  const int kind = r.kind[t];
-  const int64_t nsp = r.n_sp[dom];
-  const int32_t p0 = r.pt_off[t], p1 = r.pt_off[t + 1];
-  double* row = sample >= 0 ? r.traces + static_cast<int64_t>(sample) * r.n_cols + r.col[t] : nullptr;
-  if (kind == INSEM_Q_PARTICLE_VELOCITY) {
-    const float* u = r.su[dom];
-    const double gx = ordered_dot(r.d_dx, u, r.pts, 0, p0, p1, sw, sx);
-    const double gz = ordered_dot(r.d_dz, u, r.pts, 0, p0, p1, sw, sx);
-    if (threadIdx.x != 0) return;
-    const double ax = -gx / r.rho[t], az = -gz / r.rho[t];
-    double* pv = r.pv + 4 * t;  // prev_ax, prev_az, vel_x, vel_z
-    if (pv_mode == 1) {
-      pv[2] += half_dt * (pv[0] + ax);
-      pv[3] += half_dt * (pv[1] + az);
-    }
-    pv[0] = ax;
-    pv[1] = az;
-    if (row) {
-      row[0] = pv[2];
-      row[1] = pv[3];
-    }
-    return;
-  }
-  if (!row) return;
-  const float* field = kind == INSEM_Q_VELOCITY ? r.sv[dom] : kind == INSEM_Q_ACCELERATION ? r.sa[dom] : r.su[dom];
-  for (int c = 0; c < r.ncomp[t]; ++c) {
-    const double s = ordered_dot(r.value, field, r.pts, c * nsp, p0, p1, sw, sx);
-    if (threadIdx.x == 0) row[c] = s;
-  }
+  receiver_task<kRecWG>(r, blockIdx.x, clk, clock_step(clk), pv_mode, half_dt, sw, sx);
 }
 
 __global__ void k_zero_points(float* x, int64_t ng, const int32_t* pts, int n, int comp_mask, int ncomp) {
