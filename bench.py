@@ -10,10 +10,12 @@ coupling, sources, update, receivers), with all fields resident in HBM.
 
 Prints one JSON line with the throughput in GLL DOF-steps/s (timed over the
 production path: step chunks replayed as HIP graphs), the roofline of the
-dominant kernel (element forces of the largest elastic block) from HIP events
-around each of its launches in a second, directly launched pass of the same
-steps, and the CPU oracle timed on the host cores on a bounded sample of the
-same workload.
+dominant kernel (element forces of the elastic Curved block, with the update
+of the points it owns fused in) from HIP events around each of its launches
+in a second, directly launched pass of further steps, the HBM traffic of that
+launch measured with rocprofv3 counters (read from the committed summary
+under profiles/), and the CPU oracle timed on the host cores on a bounded
+sample (about 20 s) of the same workload.
 
 `--calibrate-copy` instead runs a plain device-to-device copy of known size
 (the reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters).
@@ -97,7 +99,7 @@ def cpu_baseline(sim, dt, max_seconds=20.0):
     t0 = time.perf_counter()
     o.advance(1)  # initialisation + first step
     first = time.perf_counter() - t0
-    n = max(2, min(200, int(max_seconds / max(first, 1e-6))))
+    n = max(2, min(20000, int(max_seconds / max(first, 1e-6))))
     t0 = time.perf_counter()
     o.advance(n)
     el = time.perf_counter() - t0
@@ -105,7 +107,7 @@ def cpu_baseline(sim, dt, max_seconds=20.0):
     o.close()
     return {"value": dof * n / el, "unit": "DOF-steps/s", "cores": int(threads), "kind": "port",
             "sample": f"{n} steps of the same 800x400 order-4 coupled model after 1 warm-up step "
-                      f"({el:.1f} s, OpenMP oracle, stepping only)"}
+                      f"({el:.1f} s, OpenMP oracle on {int(threads)} threads, stepping only)"}
 
 
 def calibrate_copy(n_bytes=1 << 30, reps=5):
