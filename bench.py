@@ -14,8 +14,10 @@ dominant kernel (element forces of the elastic Curved block, with the update
 of the points it owns fused in) from HIP events around each of its launches
 in a second, directly launched pass of further steps, the HBM traffic of that
 launch measured with rocprofv3 counters (read from the committed summary
-under profiles/), and the CPU oracle timed on the host cores on a bounded
-sample (about 20 s) of the same workload.
+under profiles/), the CPU oracle timed on the host cores on a bounded
+sample (about 20 s) of the same workload, and (under "batched") the solves/s
+and latencies of cased-hole dipole models run as batches against the same
+models run one after another on the plain path.
 
 `--calibrate-copy` instead runs a plain device-to-device copy of known size
 (the reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters).
@@ -110,6 +112,67 @@ def cpu_baseline(sim, dt, max_seconds=20.0):
                       f"({el:.1f} s, OpenMP oracle on {int(threads)} threads, stepping only)"}
 
 
+def batched_leg(n_models, duration, seed=2026):
+    """Cased-hole dipole models (azimuthal order 1) drawn from the batched
+    workload's distribution, solved end to end (host setup, power-iteration
+    Δt, all steps, traces) one after another on the plain path, then as
+    batches (one per polynomial order) on the same GPU.  Every solve is
+    queued at t = 0; a solve's latency is the time until its traces are
+    ready."""
+    import numpy as np
+
+    import insem_amd as insem
+    from insem_amd.configs import cased_hole
+
+    def draw():
+        rng = np.random.default_rng(seed)
+        return [cased_hole(rng, duration=duration) for _ in range(n_models)]
+
+    t0 = time.perf_counter()
+    plain_done, steps = [], 0
+    for sim in draw():
+        s = sim.solver
+        s.run()
+        _ = s.traces
+        steps += s.n_steps
+        s.close()
+        plain_done.append(time.perf_counter() - t0)
+    plain_s = time.perf_counter() - t0
+
+    sims = draw()
+    orders = sorted({sim.discretization.order for sim in sims})
+    t0 = time.perf_counter()
+    batch_done = [0.0] * n_models
+    groups = []
+    for order in orders:
+        idx = [i for i, sim in enumerate(sims) if sim.discretization.order == order]
+        groups.append((idx, insem.BatchSolver([sims[i] for i in idx])))
+    # the batches (one per order) run side by side, each on its own stream
+    live = list(groups)
+    while live:
+        for idx, b in live:
+            b.advance(512)
+        for idx, b in [g for g in live if g[1].step >= g[1].max_steps]:
+            for m in range(b.n_models):
+                _ = b.traces(m)
+            b.close()
+            for i in idx:
+                batch_done[i] = time.perf_counter() - t0
+        live = [g for g in live if g[1]._handle is not None]
+    batch_s = time.perf_counter() - t0
+
+    def pct(v, q):
+        return float(np.percentile(np.asarray(v) * 1e3, q))
+
+    return {"metric": "batched solves/s", "value": n_models / batch_s, "unit": "solves/s",
+            "plain_solves_per_s": n_models / plain_s, "vs_plain": plain_s / batch_s,
+            "latency_ms": {"batched_p50": pct(batch_done, 50), "batched_p99": pct(batch_done, 99),
+                           "plain_sequential_p50": pct(plain_done, 50), "plain_sequential_p99": pct(plain_done, 99)},
+            "sample": f"{n_models} cased-hole dipole models (seed {seed}, orders {orders}, {duration * 1e3:g} ms each, "
+                      f"{steps} model-steps), end to end",
+            "batches": len(orders)}
+
+
 def calibrate_copy(n_bytes=1 << 30, reps=5):
     """A plain copy of n_bytes between two device buffers, reps times."""
     import torch
@@ -130,6 +193,8 @@ def main():
     ap.add_argument("--nx", type=int, default=800)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--calibrate-copy", action="store_true", help="run only a 1 GiB device copy (counter scale)")
+    ap.add_argument("--batch-models", type=int, default=24, help="models of the batched leg (0: skip it)")
+    ap.add_argument("--batch-duration", type=float, default=5e-3, help="simulated time of each batched model (s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every run of this build uses a single MI355X "
@@ -221,7 +286,11 @@ def main():
                      "step_algorithmic_bytes": step_bytes,
                      "step_frac": step_bytes / (el / args.steps) / (HBM_PEAK_GBS * 1e9)},
     }
-    result["cpu_baseline"] = None if args.no_cpu else cpu_baseline(sim, solver.dt)
+    dt = solver.dt
+    solver.close()  # free the model's device memory before the other legs
+    if args.batch_models > 0:
+        result["batched"] = batched_leg(args.batch_models, args.batch_duration)
+    result["cpu_baseline"] = None if args.no_cpu else cpu_baseline(sim, dt)
     print(json.dumps(result), flush=True)
 
 

@@ -133,6 +133,13 @@ struct insem_hip_ctx {
   int chunk = 32;
   int use_graphs = 1;
   int fuse = 1;  // owned points updated inside the force launches
+  // batch of independent models (0: a single model)
+  int n_models = 0;
+  ModelCoef* d_models = nullptr;
+  // power iteration in flight: runs started, host start vectors kept alive
+  int lam_runs = 0;
+  std::vector<std::vector<float>> lam_host;
+  std::vector<hipGraphExec_t> lam_execs;
   std::map<int, hipGraphExec_t> graphs;
   // energy
   double* partial = nullptr;
@@ -231,8 +238,11 @@ int phys_of(int medium, int kind, int am) {
 }
 
 void free_all(insem_hip_ctx* c) {
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
   c->graphs.clear();
+  for (hipGraphExec_t g : c->lam_execs) (void)hipGraphExecDestroy(g);
+  c->lam_execs.clear();
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   for (auto& e : c->ev_force) {
@@ -710,6 +720,56 @@ void build_receivers(insem_hip_ctx* c, const insem_hip_setup* s) {
   }
 }
 
+// Model of every point, block element and receiver task of a batch, in
+// the device numbering; the coefficient table is filled by set_time_batch.
+void build_batch(insem_hip_ctx* c, const insem_hip_setup* s, const insem_hip_batch* b) {
+  c->n_models = b->n_models;
+  c->d_models = c->alloc<ModelCoef>(static_cast<size_t>(b->n_models));
+  c->energy_every = 0;
+  for (int di = 0; di < s->n_domains; ++di) {
+    DomainRT& R = c->doms[di];
+    const insem_hip_domain& d = s->domains[di];
+    const int32_t* pm = b->point_model[di];
+    std::vector<int32_t> pnew(static_cast<size_t>(R.ng));
+    for (int64_t gn = 0; gn < R.ng; ++gn) pnew[gn] = pm[R.perm_old[gn]];
+    R.dd.pmodel = c->upload(pnew);
+    R.dd.models = c->d_models;
+    const int npe = R.npe;
+    for (int bi = 0; bi < d.n_blocks; ++bi) {
+      const insem_hip_block& B = d.blocks[bi];
+      std::vector<int32_t> em(static_cast<size_t>(B.n_elements), 0);
+      for (int e = 0; e < B.n_elements; ++e) {
+        const int32_t* row = d.l2g + static_cast<size_t>(B.elements[e]) * npe;
+        int k = 0;
+        while (k < npe && row[k] < 0) ++k;
+        em[e] = pm[k < npe ? row[k] : d.row_dofs[d.row_offsets[-1 - row[0]]]];
+      }
+      BlockRT& br = R.blocks[bi];
+      br.args.emodel = c->upload(em);
+      br.args.models = c->d_models;
+    }
+  }
+  std::vector<int32_t> tm;
+  for (const TraceCol& tc : c->cols) tm.push_back(b->receiver_model[tc.receiver]);
+  c->rec.tmodel = c->upload(tm);
+  c->rec.models = c->d_models;
+}
+
+int validate_batch(const insem_hip_setup* s, const insem_hip_batch* b) {
+  if (b->n_models < 1) return set_error(INSEM_HIP_EINVAL, "a batch needs at least one model");
+  for (int di = 0; di < s->n_domains; ++di) {
+    if (!b->point_model[di]) return set_error(INSEM_HIP_EINVAL, "batch without the model of each point");
+    for (int32_t g = 0; g < s->domains[di].n_global; ++g)
+      if (b->point_model[di][g] < 0 || b->point_model[di][g] >= b->n_models)
+        return set_error(INSEM_HIP_EINVAL, "point model out of range");
+  }
+  if (s->n_receivers > 0 && !b->receiver_model) return set_error(INSEM_HIP_EINVAL, "batch without receiver models");
+  for (int i = 0; i < s->n_receivers; ++i)
+    if (b->receiver_model[i] < 0 || b->receiver_model[i] >= b->n_models)
+      return set_error(INSEM_HIP_EINVAL, "receiver model out of range");
+  return INSEM_HIP_OK;
+}
+
 // ------------------------------------------------------------ step pieces
 
 void timed_begin(insem_hip_ctx* c, hipEvent_t* e) {
@@ -726,7 +786,6 @@ void timed_end(insem_hip_ctx* c, hipEvent_t b, bool force) {
   if (force) c->ev_force_tag.push_back(c->cur_tag);
 }
 
-// Update of the owned points fused into a time step's force launches.
 // Parameters of the force launches of a time step (owned points updated,
 // grouped forces written in the group layout).
 StepParams fused_params(const StepCoef& sc, int corr, int pred, const StepClock& clk, double* partial) {
@@ -844,7 +903,7 @@ void initialize(insem_hip_ctx* c) {
 
 void predictor_all(insem_hip_ctx* c) {
   const StepCoef sc = step_coef(c, true);
-  for (DomainRT& R : c->doms) HIP_TRY(launch_predictor(R.dd, sc, c->stream));
+  for (DomainRT& R : c->doms) HIP_TRY(launch_predictor(R.dd, sc, clock_at(c, 0), c->stream));
 }
 
 // The launches of k steps after *d_clock: a standalone predictor when the
@@ -910,6 +969,33 @@ int guarded(insem_hip_ctx* c, F&& f) {
   }
 }
 
+// Queues the launches of f on the context's stream, captured once as a
+// hipGraph and launched (kept until the context's next synchronising call),
+// or directly when graphs are off.
+template <class F>
+void queue_captured(insem_hip_ctx* c, F&& f) {
+  if (!c->use_graphs) {
+    f();
+    return;
+  }
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipGraph_t g = nullptr;
+  try {
+    f();
+  } catch (...) {
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIP_TRY(hipStreamEndCapture(c->stream, &g));
+  hipGraphExec_t ge = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIP_TRY(e);
+  c->lam_execs.push_back(ge);
+  HIP_TRY(hipGraphLaunch(ge, c->stream));
+}
+
 void permute_in(const DomainRT& R, const float* src, std::vector<float>& out) {
   out.resize(static_cast<size_t>(R.ncomp) * R.ng);
   for (int cc = 0; cc < R.ncomp; ++cc)
@@ -935,10 +1021,26 @@ int insem_hip_device_count(void) {
   return n;
 }
 
-int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) {
+namespace {
+int create_impl(const insem_hip_setup* s, const insem_hip_batch* batch, insem_hip_ctx** out);
+}
+
+int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) { return create_impl(s, nullptr, out); }
+
+int insem_hip_create_batch(const insem_hip_setup* s, const insem_hip_batch* batch, insem_hip_ctx** out) {
+  if (!batch) return set_error(INSEM_HIP_EINVAL, "null batch");
+  return create_impl(s, batch, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+int create_impl(const insem_hip_setup* s, const insem_hip_batch* batch, insem_hip_ctx** out) {
   if (!out) return set_error(INSEM_HIP_EINVAL, "null output pointer");
   *out = nullptr;
-  const int v = validate_setup(s);
+  int v = validate_setup(s);
+  if (v == INSEM_HIP_OK && batch) v = validate_batch(s, batch);
   if (v != INSEM_HIP_OK) return v;
   if (insem_hip_device_count() <= 0) return set_error(INSEM_HIP_EDEVICE, "no HIP device is visible");
   auto* c = new insem_hip_ctx();
@@ -961,6 +1063,7 @@ int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) {
     c->n_sources = s->n_sources;
     c->record_every = s->record_every;
     c->energy_every = s->energy_every;
+    if (batch) build_batch(c, s, batch);
     int rows = 0;
     for (const DomainRT& R : c->doms) {
       rows += tail_blocks(R.ng) + tail_blocks(R.n_sp) + 2;  // grouped + other regular + special
@@ -971,7 +1074,7 @@ int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) {
     int dot_rows = 0;
     for (const DomainRT& R : c->doms) dot_rows = std::max(dot_rows, tail_blocks(R.ncomp * R.ng));
     c->partial = c->alloc<double>(3 * static_cast<size_t>(std::max(rows, dot_rows)) + 16);
-    c->scratch = c->alloc<double>(64);
+    c->scratch = c->alloc<double>(128);
     c->d_clock = c->alloc<int32_t>(1);
     HIP_TRY(hipMemset(c->d_clock, 0, sizeof(int32_t)));
     if (const char* g = std::getenv("INSEM_HIP_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
@@ -990,6 +1093,29 @@ int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) {
   return INSEM_HIP_OK;
 }
 
+// dt, step count and amplitude table; trace and energy buffers sized for it.
+void apply_time(insem_hip_ctx* c, double dt, int32_t n_steps, const float* amplitudes) {
+  c->dt = dt;
+  c->n_steps = n_steps;
+  const size_t na = static_cast<size_t>(c->n_sources) * (static_cast<size_t>(n_steps) + 1);
+  c->amps = c->upload(amplitudes ? amplitudes : nullptr, amplitudes ? na : 0);
+  for (DomainRT& R : c->doms) {
+    R.sp.amps = c->amps;
+    R.sp.amp_stride = n_steps + 1;
+  }
+  c->max_samples = n_steps / c->record_every + 1;
+  c->traces = c->alloc<double>(static_cast<size_t>(c->max_samples) * std::max(1, c->rec.n_cols));
+  HIP_TRY(hipMemset(c->traces, 0, static_cast<size_t>(c->max_samples) * std::max(1, c->rec.n_cols) * sizeof(double)));
+  c->rec.traces = c->traces;
+  c->max_energy_rows = c->energy_every > 0 ? n_steps / c->energy_every + 1 : 0;
+  c->energy_rows = c->alloc<double>(2 * static_cast<size_t>(std::max(1, c->max_energy_rows)));
+  c->have_time = 1;
+}
+
+}  // namespace
+
+extern "C" {
+
 int insem_hip_destroy(insem_hip_ctx* c) {
   if (!c) return INSEM_HIP_OK;
   (void)hipSetDevice(c->device);
@@ -1001,24 +1127,35 @@ int insem_hip_destroy(insem_hip_ctx* c) {
 
 int insem_hip_set_time(insem_hip_ctx* c, double dt, int32_t n_steps, const float* amplitudes) {
   if (c && c->initialized) return set_error(INSEM_HIP_ESTATE, "the time step must be set before the first step");
+  if (c && c->n_models) return set_error(INSEM_HIP_ESTATE, "a batch takes its time steps from insem_hip_set_time_batch");
   if (!(dt > 0.0) || n_steps < 0) return set_error(INSEM_HIP_EINVAL, "invalid time step or step count");
   if (c && c->n_sources > 0 && !amplitudes) return set_error(INSEM_HIP_EINVAL, "source amplitudes are required");
+  return guarded(c, [&] { apply_time(c, dt, n_steps, amplitudes); });
+}
+
+int insem_hip_set_time_batch(insem_hip_ctx* c, const double* dt, const int32_t* n_steps, const float* amplitudes) {
+  if (!c) return set_error(INSEM_HIP_EINVAL, "null context");
+  if (!c->n_models) return set_error(INSEM_HIP_ESTATE, "not a batch context");
+  if (c->initialized) return set_error(INSEM_HIP_ESTATE, "the time step must be set before the first step");
+  if (!dt || !n_steps) return set_error(INSEM_HIP_EINVAL, "null time steps or step counts");
+  if (c->n_sources > 0 && !amplitudes) return set_error(INSEM_HIP_EINVAL, "source amplitudes are required");
+  int32_t max_n = 0;
+  std::vector<ModelCoef> mc(static_cast<size_t>(c->n_models));
+  for (int m = 0; m < c->n_models; ++m) {
+    if (!(dt[m] > 0.0) || n_steps[m] < 0) return set_error(INSEM_HIP_EINVAL, "invalid time step or step count of model %d", m);
+    // the single-model expressions (step_coef)
+    mc[m].rdt = static_cast<float>(dt[m]);
+    mc[m].hdt = static_cast<float>(0.5 * dt[m]);
+    mc[m].hdt2 = static_cast<float>(0.5 * dt[m] * dt[m]);
+    mc[m].h_damp = 0.5 * dt[m];
+    mc[m].cc = dt[m] * dt[m] / 8.0;
+    mc[m].half_dt = 0.5 * dt[m];
+    mc[m].n_steps = n_steps[m];
+    max_n = std::max(max_n, n_steps[m]);
+  }
   return guarded(c, [&] {
-    c->dt = dt;
-    c->n_steps = n_steps;
-    const size_t na = static_cast<size_t>(c->n_sources) * (static_cast<size_t>(n_steps) + 1);
-    c->amps = c->upload(amplitudes ? amplitudes : nullptr, amplitudes ? na : 0);
-    for (DomainRT& R : c->doms) {
-      R.sp.amps = c->amps;
-      R.sp.amp_stride = n_steps + 1;
-    }
-    c->max_samples = n_steps / c->record_every + 1;
-    c->traces = c->alloc<double>(static_cast<size_t>(c->max_samples) * std::max(1, c->rec.n_cols));
-    HIP_TRY(hipMemset(c->traces, 0, static_cast<size_t>(c->max_samples) * std::max(1, c->rec.n_cols) * sizeof(double)));
-    c->rec.traces = c->traces;
-    c->max_energy_rows = c->energy_every > 0 ? n_steps / c->energy_every + 1 : 0;
-    c->energy_rows = c->alloc<double>(2 * static_cast<size_t>(std::max(1, c->max_energy_rows)));
-    c->have_time = 1;
+    HIP_TRY(hipMemcpy(c->d_models, mc.data(), mc.size() * sizeof(ModelCoef), hipMemcpyHostToDevice));
+    apply_time(c, dt[0], max_n, amplitudes);
   });
 }
 
@@ -1174,63 +1311,65 @@ int insem_hip_block_forces(insem_hip_ctx* c, int32_t di, int32_t bi, const float
 }
 
 // Power iteration (solver.cpp:583-667) with FP64 reductions on the device.
-int insem_hip_lambda_max(insem_hip_ctx* c, const float* start, double* lambda_out) {
-  if (!start || !lambda_out) return set_error(INSEM_HIP_EINVAL, "null argument");
+int insem_hip_lambda_start(insem_hip_ctx* c, const float* start) {
+  if (!start) return set_error(INSEM_HIP_EINVAL, "null argument");
   if (c && c->initialized) return set_error(INSEM_HIP_ESTATE, "the power iteration runs before the first step");
+  if (c && c->n_models) return set_error(INSEM_HIP_ESTATE, "the power iteration is not available on a batch");
   return guarded(c, [&] {
     hipStream_t st = c->stream;
-    double* state = c->scratch;       // [0] lambda, [1] done, [2] scale
-    double* out3 = c->scratch + 8;    // reductions
-    double* nxy = c->scratch + 16;    // coupled: [0] nx, [1] ny
-    double best = 0.0;
+    // scratch: per run r (each domain, then the coupled pair) [32 r + 0..2]
+    // = lambda, done, scale; [32 r + 8..10] reductions; [32 r + 16..17] nx, ny
+    c->lam_runs = 0;
+    c->lam_host.clear();
     size_t off = 0;
     for (size_t di = 0; di < c->doms.size(); ++di) {
       DomainRT& R = c->doms[di];
+      double* state = c->scratch + 32 * c->lam_runs;
+      double* out3 = state + 8;
       const size_t len = static_cast<size_t>(R.ncomp) * R.ng;
-      std::vector<float> h;
-      permute_in(R, start + off, h);
+      c->lam_host.emplace_back();
+      permute_in(R, start + off, c->lam_host.back());
       off += len;
-      HIP_TRY(hipMemcpyAsync(R.tmp, h.data(), len * sizeof(float), hipMemcpyHostToDevice, st));
-      const double init[3] = {0.0, 0.0, 0.0};
-      HIP_TRY(hipMemcpyAsync(state, init, sizeof init, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemcpyAsync(R.tmp, c->lam_host.back().data(), len * sizeof(float), hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemsetAsync(state, 0, 3 * sizeof(double), st));
       const int all = (1 << R.ncomp) - 1;
-      for (int it = 0; it < 100; ++it) {
-        HIP_TRY(launch_zero_points(R.tmp, R.ng, R.d_dir, R.n_dir, all, R.ncomp, st));
-        HIP_TRY(launch_zero_points(R.tmp, R.ng, R.d_axis, R.n_axis, R.axis_mask, R.ncomp, st));
-        run_forces(c, R, R.tmp, c->stream);
-        HIP_TRY(launch_assemble(R.dd, st));
-        int nb = 0;
-        HIP_TRY(launch_power_stats(R.dd, R.tmp, c->partial, state, &nb, st));
-        HIP_TRY(launch_reduce(c->partial, nb, 3, out3, 0, st));
-        HIP_TRY(launch_power_update(out3, state, st));
-        HIP_TRY(launch_scale(R.tmp, static_cast<int64_t>(len), state, st));
-      }
-      double lam = 0.0;
-      HIP_TRY(hipMemcpyAsync(&lam, state, sizeof(double), hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
-      best = std::max(best, lam);
+      queue_captured(c, [&] {
+        for (int it = 0; it < 100; ++it) {
+          HIP_TRY(launch_zero_points(R.tmp, R.ng, R.d_dir, R.n_dir, all, R.ncomp, st));
+          HIP_TRY(launch_zero_points(R.tmp, R.ng, R.d_axis, R.n_axis, R.axis_mask, R.ncomp, st));
+          run_forces(c, R, R.tmp, st);
+          HIP_TRY(launch_assemble(R.dd, st));
+          int nb = 0;
+          HIP_TRY(launch_power_stats(R.dd, R.tmp, c->partial, state, &nb, st));
+          HIP_TRY(launch_reduce(c->partial, nb, 3, out3, 0, st));
+          HIP_TRY(launch_power_update(out3, state, st));
+          HIP_TRY(launch_scale(R.tmp, static_cast<int64_t>(len), state, st));
+        }
+      });
+      ++c->lam_runs;
     }
     if (c->solid >= 0 && c->fluid >= 0 && c->doms[c->solid].n_cpl > 0) {
       DomainRT& S = c->doms[c->solid];
       DomainRT& F = c->doms[c->fluid];
       DomainRT* order[2] = {&S, &F};
-      // Save u and a; start vectors solid u, then fluid u.
-      std::vector<std::vector<float>> saved;
-      for (DomainRT* R : order)
-        for (float* p : {R->dd.u, R->dd.a}) {
-          std::vector<float> h(static_cast<size_t>(R->ncomp) * R->ng);
-          HIP_TRY(hipMemcpy(h.data(), p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
-          saved.push_back(std::move(h));
-        }
+      double* state = c->scratch + 32 * c->lam_runs;
+      double* nxy = state + 16;
+      // Keep u and a (device copies: tmp holds u, fl_tmp a); start vectors
+      // solid u, then fluid u.
       for (DomainRT* R : order) {
         const size_t len = static_cast<size_t>(R->ncomp) * R->ng;
-        std::vector<float> h;
-        permute_in(*R, start + off, h);
-        off += len;
-        HIP_TRY(hipMemcpyAsync(R->dd.u, h.data(), len * sizeof(float), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(R->tmp, R->dd.u, len * sizeof(float), hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(R->fl_tmp, R->dd.a, len * sizeof(float), hipMemcpyDeviceToDevice, st));
       }
-      const double init[3] = {0.0, 0.0, 0.0};
-      HIP_TRY(hipMemcpyAsync(state, init, sizeof init, hipMemcpyHostToDevice, st));
+      for (DomainRT* R : order) {
+        const size_t len = static_cast<size_t>(R->ncomp) * R->ng;
+        c->lam_host.emplace_back();
+        permute_in(*R, start + off, c->lam_host.back());
+        off += len;
+        HIP_TRY(hipMemcpyAsync(R->dd.u, c->lam_host.back().data(), len * sizeof(float), hipMemcpyHostToDevice, st));
+      }
+      HIP_TRY(hipMemsetAsync(state, 0, 3 * sizeof(double), st));
+      queue_captured(c, [&] {
       for (int it = 0; it < 120; ++it) {
         for (DomainRT* R : order) {
           HIP_TRY(launch_zero_points(R->dd.u, R->ng, R->d_dir, R->n_dir, (1 << R->ncomp) - 1, R->ncomp, st));
@@ -1260,19 +1399,37 @@ int insem_hip_lambda_max(insem_hip_ctx* c, const float* start, double* lambda_ou
         for (DomainRT* R : order)
           HIP_TRY(launch_scale_from(R->dd.u, R->dd.a, static_cast<int64_t>(R->ncomp) * R->ng, state, st));
       }
-      double lam = 0.0;
-      HIP_TRY(hipMemcpyAsync(&lam, state, sizeof(double), hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
-      int k = 0;
-      for (DomainRT* R : order)
-        for (float* p : {R->dd.u, R->dd.a}) {
-          HIP_TRY(hipMemcpy(p, saved[k].data(), saved[k].size() * sizeof(float), hipMemcpyHostToDevice));
-          ++k;
-        }
-      best = std::max(best, lam);
+      });
+      for (DomainRT* R : order) {
+        const size_t len = static_cast<size_t>(R->ncomp) * R->ng;
+        HIP_TRY(hipMemcpyAsync(R->dd.u, R->tmp, len * sizeof(float), hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(R->dd.a, R->fl_tmp, len * sizeof(float), hipMemcpyDeviceToDevice, st));
+      }
+      ++c->lam_runs;
     }
+  });
+}
+
+int insem_hip_lambda_result(insem_hip_ctx* c, double* lambda_out) {
+  if (!lambda_out) return set_error(INSEM_HIP_EINVAL, "null argument");
+  if (c && c->lam_runs == 0) return set_error(INSEM_HIP_ESTATE, "insem_hip_lambda_start has not been called");
+  return guarded(c, [&] {
+    std::vector<double> st(32 * static_cast<size_t>(c->lam_runs));
+    HIP_TRY(hipMemcpyAsync(st.data(), c->scratch, st.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double best = 0.0;
+    for (int r = 0; r < c->lam_runs; ++r) best = std::max(best, st[32 * static_cast<size_t>(r)]);
+    c->lam_host.clear();
+    for (hipGraphExec_t g : c->lam_execs) (void)hipGraphExecDestroy(g);
+    c->lam_execs.clear();
     *lambda_out = best;
   });
+}
+
+int insem_hip_lambda_max(insem_hip_ctx* c, const float* start, double* lambda_out) {
+  if (!lambda_out) return set_error(INSEM_HIP_EINVAL, "null argument");
+  const int rc = insem_hip_lambda_start(c, start);
+  return rc != INSEM_HIP_OK ? rc : insem_hip_lambda_result(c, lambda_out);
 }
 
 int insem_hip_synchronize(insem_hip_ctx* c) {

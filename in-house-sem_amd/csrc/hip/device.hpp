@@ -90,6 +90,17 @@ __device__ __forceinline__ void group_locate(const Groups& G, int64_t g, int64_t
   *cnt = k;
 }
 
+// Time-step coefficients of one model of a batch, computed as a single
+// model's are (float(dt), float(dt/2), float(dt^2/2), dt/2, dt^2/8).
+struct ModelCoef {
+  float rdt = 0.0f, hdt = 0.0f, hdt2 = 0.0f, pad = 0.0f;
+  double h_damp = 0.0;   // dt/2: Stacey implicit weight
+  double cc = 0.0;       // dt^2/8: kinetic energy
+  double half_dt = 0.0;  // particle-velocity trapezoid
+  int32_t n_steps = 0;   // the model's last step
+  int32_t pad2 = 0;
+};
+
 // Arguments of one element-force launch (one block of one domain).
 struct ForceArgs {
   int nb = 0;
@@ -130,6 +141,9 @@ struct ForceArgs {
   Groups grp;
   const uint8_t* rank = nullptr;  // [nb][npe]
   float* contrib = nullptr;
+  // Batches: model of each block element and the models' coefficients.
+  const int32_t* emodel = nullptr;
+  const ModelCoef* models = nullptr;
 };
 
 // Per-launch parameters of the element-force kernels.
@@ -164,6 +178,10 @@ struct DomDev {
   const float* fl = nullptr;
   Groups grp;                       // grouped points (fused steps)
   const float* contrib = nullptr;
+  // Batches: model of each point and the models' coefficients (null for a
+  // single model, whose coefficients come with the launch).
+  const int32_t* pmodel = nullptr;
+  const ModelCoef* models = nullptr;
 };
 
 // Special-point data of one domain ([0, n_sp) = points n_reg .. ng - 1).

@@ -448,13 +448,16 @@ struct Slot {
   float pim = 0.0f;
 };
 
-// First round trip: the slot's point, operands and rank.
+// First round trip: the slot's point, operands and rank.  In a time step
+// of a batch, elements of models past their last step are skipped.
 template <int NG, int PHYS, bool RECT>
-__device__ __forceinline__ void stage_point(const ForceArgs& A, int batch, int le, int k, Slot<Shape<PHYS>::C>& s) {
+__device__ __forceinline__ void stage_point(const ForceArgs& A, const StepParams& P, int step, int batch, int le,
+                                            int k, Slot<Shape<PHYS>::C>& s) {
   constexpr int NPE = NG * NG;
   constexpr int EPB = kWG / NPE;
   const int b = batch * EPB + le;
   s.act = le < EPB && b < A.nb;
+  if (s.act && P.fuse && A.models) s.act = step <= A.models[A.emodel[b]].n_steps;
   s.b = b;
   if (s.act) {
     s.m = A.l2g[static_cast<size_t>(b) * NPE + k];
@@ -491,13 +494,26 @@ __device__ __forceinline__ void stage_gather(const ForceArgs& A, int fuse, Slot<
 
 // Consumes the slot's force fo.
 template <int C, int NPE>
-__device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepParams& P, const Slot<C>& s, int k,
-                                              const float* U, const float* fo, bool energy, double& ep, double& ek) {
+__device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepParams& P, int step, const Slot<C>& s,
+                                              int k, const float* U, const float* fo, bool energy, double& ep,
+                                              double& ek) {
   if (s.owned) {
     // The slot's force is the point's assembled force (0.0f + f, the
     // assembly's sum of one term); the update follows the tail's
-    // expressions in the same order (update.hip).
+    // expressions in the same order (update.hip), with the model's own
+    // coefficients on a batch (whose last step predicts nothing).
     const int64_t g = s.m;
+    float rdt = P.rdt, hdt = P.hdt, hdt2 = P.hdt2;
+    double cc = P.cc;
+    int pred = P.pred;
+    if (A.models) {
+      const ModelCoef& mc = A.models[A.emodel[s.b]];
+      rdt = mc.rdt;
+      hdt = mc.hdt;
+      hdt2 = mc.hdt2;
+      cc = mc.cc;
+      pred = pred && step < mc.n_steps;
+    }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int64_t o = c * A.ng + g;
@@ -505,14 +521,14 @@ __device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepPara
       const float f = 0.0f + fo[c];
       const float a = f * s.pim;
       if (energy) ep += static_cast<double>(u) * static_cast<double>(f);
-      if (P.corr) v = v + P.hdt * a;
+      if (P.corr) v = v + hdt * a;
       if (energy) {
         const double vd = v, ad = a;
-        ek += static_cast<double>(A.mass[g]) * (0.5 * vd * vd - P.cc * ad * ad);
+        ek += static_cast<double>(A.mass[g]) * (0.5 * vd * vd - cc * ad * ad);
       }
-      if (P.pred) {
-        u = u + (P.rdt * v + P.hdt2 * a);
-        v = v + P.hdt * a;
+      if (pred) {
+        u = u + (rdt * v + hdt2 * a);
+        v = v + hdt * a;
         A.uw[o] = u;
       } else {
         A.a[o] = a;
@@ -537,8 +553,8 @@ __device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepPara
 
 // One batch of a block from the global field.
 template <int NG, int PHYS, bool RECT>
-__device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams& P, Tiles<NG, PHYS>& T, int batch,
-                                            bool energy, double& ep, double& ek) {
+__device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams& P, int step, Tiles<NG, PHYS>& T,
+                                            int batch, bool energy, double& ep, double& ek) {
   constexpr int NPE = NG * NG;
   constexpr int C = Shape<PHYS>::C;
   const int t = threadIdx.x;
@@ -547,7 +563,7 @@ __device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams
   const int i = k % NG, j = k / NG;
   stage_matrices(A, T);
   Slot<C> s;
-  stage_point<NG, PHYS, RECT>(A, batch, le, k, s);
+  stage_point<NG, PHYS, RECT>(A, P, step, batch, le, k, s);
   stage_gather<C>(A, P.fuse, s);
   float* U = T.u + le * C * NPE;
   if (s.act) {
@@ -563,7 +579,7 @@ __device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams
   if (s.act) {
     float fo[C];
     element_out<NG, PHYS>(T, s.o.axial, le, k, i, j, xc, yc, direct0, direct1, fo);
-    slot_epilogue<C, NPE>(A, P, s, k, U, fo, energy, ep, ek);
+    slot_epilogue<C, NPE>(A, P, step, s, k, U, fo, energy, ep, ek);
   }
 }
 
@@ -609,7 +625,7 @@ __global__ __launch_bounds__(kWG) void k_forces(ForceArgs A, StepParams P, Recei
     float* sx = reinterpret_cast<float*>(smem + kWG * sizeof(double));
     receiver_task<kWG>(R, wg, P.clk, step - 1, 1, half_dt, sw, sx);
   } else {
-    force_batch<NG, PHYS, RECT>(A, P, *reinterpret_cast<Tiles<NG, PHYS>*>(smem),
+    force_batch<NG, PHYS, RECT>(A, P, step, *reinterpret_cast<Tiles<NG, PHYS>*>(smem),
                                 xcd_contiguous(wg - n_rec, gridDim.x - n_rec), energy, ep, ek);
   }
   if (energy) write_energy(ep, ek, red, P.partial + 2 * blockIdx.x);

@@ -27,6 +27,9 @@ struct ReceiverDev {
   const float* sv[2] = {nullptr, nullptr};
   const float* sa[2] = {nullptr, nullptr};
   int64_t n_sp[2] = {0, 0};
+  // Batches: model of each task and the models' coefficients.
+  const int32_t* tmodel = nullptr;
+  const ModelCoef* models = nullptr;
 };
 
 // Element forces of one block (gather = read the global field through l2g;
@@ -64,8 +67,9 @@ hipError_t launch_tail(const DomDev& d, const SpecialDev& sp, const StepCoef& sc
 hipError_t launch_energy_row(const double* partial, int n, const StepClock& clk, double* rows, hipStream_t s);
 // *base += n (end of a chunk of n steps).
 hipError_t launch_clock_add(int32_t* base, int n, hipStream_t s);
-// u += rdt v + hdt2 a; v += hdt a over all points.
-hipError_t launch_predictor(const DomDev& d, const StepCoef& sc, hipStream_t s);
+// u += rdt v + hdt2 a; v += hdt a over all points, for the step after the
+// clock's (per-model coefficients on a batch).
+hipError_t launch_predictor(const DomDev& d, const StepCoef& sc, const StepClock& clk, hipStream_t s);
 // f = assembly of f_local over all points (no update).
 hipError_t launch_assemble(const DomDev& d, hipStream_t s);
 // Fixed-order sum of n rows of `width` doubles into out (+= when accumulate).

@@ -43,6 +43,12 @@ __device__ double ordered_dot(const float* w, const float* x, const int32_t* pts
 template <int NT>
 __device__ void receiver_task(const ReceiverDev& r, int t, const StepClock& clk, int step, int pv_mode,
                               double half_dt, double* sw, float* sx) {
+  if (r.models) {
+    // a batch: the task's own model's step size, nothing past its last step
+    const ModelCoef& m = r.models[r.tmodel[t]];
+    if (step > m.n_steps) return;
+    half_dt = m.half_dt;
+  }
   const int sample =
       step % clk.record_every == 0 && step / clk.record_every < clk.max_samples ? step / clk.record_every : -1;
   const int dom = r.domain[t];

@@ -53,11 +53,35 @@ __device__ __forceinline__ void gather_sum(const DomDev& d, int64_t g, float* f)
   }
 }
 
+// Coefficients of point g at `step`: the launch's, or on a batch its
+// model's.  A batch model past its last step is inactive, and its last step
+// predicts nothing (as a run of that length alone).
+struct PointStep {
+  float rdt, hdt, hdt2;
+  double cc;
+  int pred;
+  bool active;
+};
+
+__device__ __forceinline__ PointStep point_step(const DomDev& d, const StepCoef& sc, int pred, int step, int64_t g) {
+  PointStep p{sc.rdt, sc.hdt, sc.hdt2, sc.cc, pred, true};
+  if (d.models) {
+    const ModelCoef& m = d.models[d.pmodel[g]];
+    p.rdt = m.rdt;
+    p.hdt = m.hdt;
+    p.hdt2 = m.hdt2;
+    p.cc = m.cc;
+    p.active = step <= m.n_steps;
+    p.pred = pred && step < m.n_steps;
+  }
+  return p;
+}
+
 // a = M^-1 f, optional corrector, energy terms and the next predictor at a
 // regular point g with assembled force f.
 template <int C>
-__device__ __forceinline__ void update_point(const DomDev& d, const StepCoef& sc, int corr, int pred, bool energy,
-                                             int64_t g, const float* f, double& ep, double& ek) {
+__device__ __forceinline__ void update_point(const DomDev& d, const PointStep& ps, int corr, bool energy, int64_t g,
+                                             const float* f, double& ep, double& ek) {
   const float im = d.im[g];
 #pragma unroll
   for (int c = 0; c < C; ++c) {
@@ -65,16 +89,16 @@ __device__ __forceinline__ void update_point(const DomDev& d, const StepCoef& sc
     float u = d.u[o], v = d.v[o];
     const float a = f[c] * im;
     if (energy) ep += static_cast<double>(u) * static_cast<double>(f[c]);
-    if (corr) v = v + sc.hdt * a;
+    if (corr) v = v + ps.hdt * a;
     if (energy) {
       const double vd = v, ad = a;
-      ek += static_cast<double>(d.mass[g]) * (0.5 * vd * vd - sc.cc * ad * ad);
+      ek += static_cast<double>(d.mass[g]) * (0.5 * vd * vd - ps.cc * ad * ad);
     }
-    if (pred) {
+    if (ps.pred) {
       // The next step recomputes a at regular points before anything
       // reads it, so it is only stored when the step ends the call.
-      u = u + (sc.rdt * v + sc.hdt2 * a);
-      v = v + sc.hdt * a;
+      u = u + (ps.rdt * v + ps.hdt2 * a);
+      v = v + ps.hdt * a;
       d.u[o] = u;
     } else {
       d.a[o] = a;
@@ -97,22 +121,26 @@ __device__ __forceinline__ void write_partials(bool energy, double ep, double ek
 // the block's index within this part of the launch.
 template <int C, bool W>
 __device__ __forceinline__ void regular_body(const DomDev& d, int64_t first, const StepCoef& sc, int corr, int pred,
-                                             bool energy, int vb, double& ep, double& ek) {
+                                             int step, bool energy, int vb, double& ep, double& ek) {
   const int64_t g = first + static_cast<int64_t>(vb) * kTB + threadIdx.x;
   if (g >= d.n_reg) return;
+  const PointStep ps = point_step(d, sc, pred, step, g);
+  if (!ps.active) return;
   float f[C];
   gather_sum<C, W>(d, g, f);
-  update_point<C>(d, sc, corr, pred, energy, g, f, ep, ek);
+  update_point<C>(d, ps, corr, energy, g, f, ep, ek);
 }
 
 // Grouped points: the contributions of a point lie in rank order at stride
 // size[q] in the group layout; the sum starts from zero in slot order as the
 // assembly's does.
 template <int C>
-__device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc, int corr, int pred, bool energy,
-                                             int vb, double& ep, double& ek) {
+__device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc, int corr, int pred, int step,
+                                             bool energy, int vb, double& ep, double& ek) {
   const int64_t g = d.grp.begin + static_cast<int64_t>(vb) * kTB + threadIdx.x;
   if (g >= d.grp.end) return;
+  const PointStep ps = point_step(d, sc, pred, step, g);
+  if (!ps.active) return;
   int64_t off, n;
   int cnt;
   group_locate(d.grp, g, &off, &n, &cnt);
@@ -124,7 +152,7 @@ __device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc
     for (int r = 0; r < cnt; ++r) s += src[(c * cnt + r) * n];
     f[c] = s;
   }
-  update_point<C>(d, sc, corr, pred, energy, g, f, ep, ek);
+  update_point<C>(d, ps, corr, energy, g, f, ep, ek);
 }
 
 // Small SPD solve without pivoting, in double as the reference's
@@ -155,8 +183,10 @@ template <int C, bool W>
 __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
                                              int pred, int step, bool energy, int vb, double& ep, double& ek) {
   const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
-  if (i < sp.n_sp) {
+  const PointStep ps = i < sp.n_sp ? point_step(d, sc, pred, step, d.n_reg + i) : PointStep{};
+  if (i < sp.n_sp && ps.active) {
     const int64_t g = d.n_reg + i;
+    const double h_damp = d.models && sc.h_damp > 0.0 ? d.models[d.pmodel[g]].h_damp : sc.h_damp;
     float f[C];
     gather_sum<C, W>(d, g, f);
     float u[C], v[C], a[C];
@@ -210,7 +240,7 @@ __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& 
 #pragma unroll
         for (int s = 0; s < C; ++s) {
           rhs[r] -= static_cast<double>(cf[r * C + s]) * v[s];
-          A[r * C + s] = sc.h_damp * cf[r * C + s] + (r == s ? m : 0.0);
+          A[r * C + s] = h_damp * cf[r * C + s] + (r == s ? m : 0.0);
         }
       }
       solve_small<C>(A, rhs);
@@ -224,18 +254,18 @@ __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& 
     }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      if (corr) v[c] = v[c] + sc.hdt * a[c];
+      if (corr) v[c] = v[c] + ps.hdt * a[c];
       if (energy) {
         const double vd = v[c], ad = a[c];
-        ek += static_cast<double>(d.mass[g]) * (0.5 * vd * vd - sc.cc * ad * ad);
+        ek += static_cast<double>(d.mass[g]) * (0.5 * vd * vd - ps.cc * ad * ad);
       }
       sp.su[c * sp.n_sp + i] = u[c];
       sp.sv[c * sp.n_sp + i] = v[c];
       sp.sa[c * sp.n_sp + i] = a[c];
       const int64_t o = c * d.ng + g;
-      if (pred) {
-        d.u[o] = u[c] + (sc.rdt * v[c] + sc.hdt2 * a[c]);
-        v[c] = v[c] + sc.hdt * a[c];
+      if (ps.pred) {
+        d.u[o] = u[c] + (ps.rdt * v[c] + ps.hdt2 * a[c]);
+        v[c] = v[c] + ps.hdt * a[c];
       }
       d.v[o] = v[c];
       d.a[o] = a[c];
@@ -258,19 +288,22 @@ __global__ __launch_bounds__(kTB) void k_tail(DomDev d, SpecialDev sp, StepCoef 
   if (b < n_sp_blocks)
     special_body<C, W>(d, sp, sc, corr, pred, step, energy, b, ep, ek);
   else if (b < n_sp_blocks + n_grp_blocks)
-    grouped_body<C>(d, sc, corr, pred, energy, b - n_sp_blocks, ep, ek);
+    grouped_body<C>(d, sc, corr, pred, step, energy, b - n_sp_blocks, ep, ek);
   else
-    regular_body<C, W>(d, reg_first, sc, corr, pred, energy, b - n_sp_blocks - n_grp_blocks, ep, ek);
+    regular_body<C, W>(d, reg_first, sc, corr, pred, step, energy, b - n_sp_blocks - n_grp_blocks, ep, ek);
   write_partials(energy, ep, ek, lds, partial);
 }
 
-__global__ __launch_bounds__(kTB) void k_predictor(float* u, float* v, const float* a, int64_t n, float rdt,
-                                                   float hdt, float hdt2) {
+// Predictor of the step after the clock's (start of an advance call); on a
+// batch with each point's model coefficients, skipping finished models.
+__global__ __launch_bounds__(kTB) void k_predictor(DomDev d, StepCoef sc, StepClock clk) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
-  if (i >= n) return;
-  const float vi = v[i], ai = a[i];
-  u[i] = u[i] + (rdt * vi + hdt2 * ai);
-  v[i] = vi + hdt * ai;
+  if (i >= d.ncomp * d.ng) return;
+  const PointStep ps = point_step(d, sc, 1, clock_step(clk) + 1, i % d.ng);
+  if (!ps.active) return;
+  const float vi = d.v[i], ai = d.a[i];
+  d.u[i] = d.u[i] + (ps.rdt * vi + ps.hdt2 * ai);
+  d.v[i] = vi + ps.hdt * ai;
 }
 
 template <int C, bool W>
@@ -473,10 +506,10 @@ hipError_t launch_tail(const DomDev& d, const SpecialDev& sp, const StepCoef& sc
   return hipGetLastError();
 }
 
-hipError_t launch_predictor(const DomDev& d, const StepCoef& sc, hipStream_t s) {
+hipError_t launch_predictor(const DomDev& d, const StepCoef& sc, const StepClock& clk, hipStream_t s) {
   const int64_t n = d.ncomp * d.ng;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_predictor, dim3(blocks_for(n)), dim3(kTB), 0, s, d.u, d.v, d.a, n, sc.rdt, sc.hdt, sc.hdt2);
+  hipLaunchKernelGGL(k_predictor, dim3(blocks_for(n)), dim3(kTB), 0, s, d, sc, clk);
   return hipGetLastError();
 }
 

@@ -1,5 +1,6 @@
 #include "batch.hpp"
 
+#include <algorithm>
 #include <string>
 
 #include "error.hpp"
@@ -165,6 +166,125 @@ BatchImage union_of(const std::vector<const SetupImage*>& models) {
   img->refresh_views();
   out.image = std::move(img);
   return out;
+}
+
+BatchSolver::BatchSolver(std::vector<std::unique_ptr<SetupImage>> models) : models_(std::move(models)) {
+  INSEM_REQUIRE(!models_.empty(), "a batch needs at least one model");
+  // Each model's own power iteration on a short-lived context of it alone.
+  // A window of them is queued before the first result is read, so that
+  // the iterations of different models overlap on the device while the host
+  // queues the next ones.
+  for (const auto& m : models_) INSEM_REQUIRE(m, "null model in a batch");
+  constexpr std::size_t kWindow = 16;
+  for (std::size_t w0 = 0; w0 < models_.size(); w0 += kWindow) {
+    const std::size_t w1 = std::min(models_.size(), w0 + kWindow);
+    std::vector<insem_hip_ctx*> ctx(w1 - w0, nullptr);
+    try {
+      for (std::size_t i = w0; i < w1; ++i) {
+        check_hip(insem_hip_create(&models_[i]->view, &ctx[i - w0]));
+        const std::vector<float> start = models_[i]->power_start_vectors();
+        check_hip(insem_hip_lambda_start(ctx[i - w0], start.data()));
+      }
+      for (std::size_t i = w0; i < w1; ++i) {
+        double lambda = 0.0;
+        check_hip(insem_hip_lambda_result(ctx[i - w0], &lambda));
+        time_.push_back(model_time(models_[i]->time, lambda));
+        max_steps_ = std::max(max_steps_, time_.back().n_steps);
+      }
+    } catch (...) {
+      for (insem_hip_ctx* c : ctx) insem_hip_destroy(c);
+      throw;
+    }
+    for (insem_hip_ctx* c : ctx) insem_hip_destroy(c);
+  }
+  std::vector<const SetupImage*> ptrs;
+  for (const auto& m : models_) ptrs.push_back(m.get());
+  union_ = union_of(ptrs);
+  const SetupImage& U = *union_.image;
+  // each model's source rows at its own dt, padded to the union's length
+  const std::size_t len = static_cast<std::size_t>(max_steps_) + 1;
+  std::vector<float> amps(U.sources.size() * len, 0.0f);
+  std::size_t row = 0;
+  std::vector<double> dts;
+  std::vector<int32_t> steps;
+  for (std::size_t mi = 0; mi < models_.size(); ++mi) {
+    const std::vector<float> a = models_[mi]->source_amplitudes(time_[mi].dt, time_[mi].n_steps);
+    const std::size_t n = static_cast<std::size_t>(time_[mi].n_steps) + 1;
+    for (std::size_t s = 0; s < models_[mi]->sources.size(); ++s, ++row)
+      std::copy(a.begin() + s * n, a.begin() + (s + 1) * n, amps.begin() + row * len);
+    dts.push_back(time_[mi].dt);
+    steps.push_back(time_[mi].n_steps);
+  }
+  insem_hip_batch b{};
+  b.n_models = n_models();
+  for (std::size_t di = 0; di < union_.point_model.size() && di < 2; ++di) b.point_model[di] = union_.point_model[di].data();
+  b.receiver_model = union_.receiver_model.data();
+  check_hip(insem_hip_create_batch(&U.view, &b, &ctx_));
+  try {
+    check_hip(insem_hip_set_time_batch(ctx_, dts.data(), steps.data(), amps.empty() ? nullptr : amps.data()));
+  } catch (...) {
+    insem_hip_destroy(ctx_);
+    ctx_ = nullptr;
+    throw;
+  }
+}
+
+BatchSolver::~BatchSolver() {
+  if (ctx_) insem_hip_destroy(ctx_);
+}
+
+int BatchSolver::step() const {
+  int32_t s = 0;
+  check_hip(insem_hip_step(ctx_, &s));
+  return s;
+}
+
+void BatchSolver::advance(int n) { check_hip(insem_hip_advance(ctx_, n)); }
+
+int BatchSolver::n_samples(int m) const {
+  int32_t ns = 0;
+  check_hip(insem_hip_n_samples(ctx_, &ns));
+  if (ns == 0) return 0;
+  const int own = time_.at(m).n_steps / union_.image->time.record_every + 1;
+  return std::min(static_cast<int>(ns), own);
+}
+
+std::vector<Solver::Trace> BatchSolver::traces(int m) const {
+  int32_t ns = 0;
+  check_hip(insem_hip_n_samples(ctx_, &ns));
+  const SetupImage& U = *union_.image;
+  std::size_t total = 0;
+  for (const auto& r : U.receivers)
+    for (int nc : r.ncomp) total += static_cast<std::size_t>(nc) * ns;
+  std::vector<double> all(total + 1);
+  check_hip(insem_hip_read_traces(ctx_, all.data()));
+  const int keep = n_samples(m);
+  std::vector<Solver::Trace> out;
+  std::size_t o = 0;
+  const int r0 = union_.receiver_offset.at(m), r1 = union_.receiver_offset.at(m + 1);
+  for (int ri = 0; ri < static_cast<int>(U.receivers.size()); ++ri) {
+    const auto& r = U.receivers[ri];
+    const auto& own = models_[m]->receivers;
+    for (std::size_t q = 0; q < r.quantities.size(); ++q) {
+      const std::size_t n = static_cast<std::size_t>(r.ncomp[q]) * ns;
+      if (ri >= r0 && ri < r1) {
+        Solver::Trace t;
+        t.receiver = own[ri - r0].name;
+        t.quantity = r.quantities[q];
+        t.ncomp = r.ncomp[q];
+        t.data.assign(all.begin() + o, all.begin() + o + static_cast<std::size_t>(t.ncomp) * keep);
+        out.push_back(std::move(t));
+      }
+      o += n;
+    }
+  }
+  return out;
+}
+
+int64_t BatchSolver::footprint() const {
+  int64_t b = 0;
+  check_hip(insem_hip_footprint(ctx_, &b));
+  return b;
 }
 
 }  // namespace insem_amd

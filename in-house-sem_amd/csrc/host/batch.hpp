@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "model.hpp"
+#include "solver.hpp"
 
 namespace insem_amd {
 
@@ -28,5 +29,39 @@ struct BatchImage {
 // All models must share the polynomial order, the coordinate system and
 // azimuthal order, the media present and record_every.
 BatchImage union_of(const std::vector<const SetupImage*>& models);
+
+// Many models stepped as one union on the device (insem_hip_create_batch).
+// Each model's Δt comes from its own power iteration (a short-lived
+// single-model context), as a Solver of that model alone would get it.
+class BatchSolver {
+ public:
+  explicit BatchSolver(std::vector<std::unique_ptr<SetupImage>> models);
+  ~BatchSolver();
+  BatchSolver(const BatchSolver&) = delete;
+  BatchSolver& operator=(const BatchSolver&) = delete;
+
+  int n_models() const { return static_cast<int>(models_.size()); }
+  const SetupImage& model(int m) const { return *models_.at(m); }
+  double dt(int m) const { return time_.at(m).dt; }
+  double stable_dt(int m) const { return time_.at(m).stable_dt; }
+  int n_steps(int m) const { return time_.at(m).n_steps; }
+  int max_steps() const { return max_steps_; }
+  int step() const;
+  void advance(int n);
+  void run() { advance(max_steps_ - step()); }
+  // Samples of model m so far (its own record cadence, up to its n_steps).
+  int n_samples(int m) const;
+  // Model m's traces, as Solver::traces of that model alone.
+  std::vector<Solver::Trace> traces(int m) const;
+  insem_hip_ctx* context() const { return ctx_; }
+  int64_t footprint() const;
+
+ private:
+  std::vector<std::unique_ptr<SetupImage>> models_;
+  BatchImage union_;
+  std::vector<ModelTime> time_;
+  int max_steps_ = 0;
+  insem_hip_ctx* ctx_ = nullptr;
+};
 
 }  // namespace insem_amd

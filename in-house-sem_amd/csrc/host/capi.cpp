@@ -13,6 +13,7 @@
 #include "basis.hpp"
 #include "error.hpp"
 #include "model.hpp"
+#include "batch.hpp"
 #include "solver.hpp"
 
 using namespace insem_amd;
@@ -446,6 +447,67 @@ int insem_solver_energy(void* h, double* out) {
     for (std::size_t i = 0; i < e.size(); ++i)
       for (int k = 0; k < 3; ++k) out[3 * i + k] = e[i][k];
   });
+}
+
+// ---------------------------------------------------------------- batch
+
+// Takes ownership of the n images.
+void* insem_batch_new(void* const* images, int32_t n) {
+  return guard_ptr([&]() -> void* {
+    std::vector<std::unique_ptr<SetupImage>> v;
+    for (int32_t i = 0; i < n; ++i) v.emplace_back(static_cast<SetupImage*>(images[i]));
+    return new BatchSolver(std::move(v));
+  });
+}
+
+void insem_batch_free(void* h) { delete static_cast<BatchSolver*>(h); }
+
+void* insem_batch_context(void* h) { return static_cast<BatchSolver*>(h)->context(); }
+
+void* insem_batch_model_image(void* h, int32_t m) {
+  return const_cast<SetupImage*>(&static_cast<BatchSolver*>(h)->model(m));
+}
+
+// d: dt, stable_dt; i: n_steps, samples so far.  m = -1: n_models, max steps, union step.
+int insem_batch_info(void* h, int32_t m, double* d, int32_t* i) {
+  return guard([&] {
+    const BatchSolver& b = *static_cast<BatchSolver*>(h);
+    if (m < 0) {
+      i[0] = b.n_models();
+      i[1] = b.max_steps();
+      i[2] = b.step();
+      return;
+    }
+    INSEM_REQUIRE(m < b.n_models(), "no model " << m << " in the batch");
+    d[0] = b.dt(m);
+    d[1] = b.stable_dt(m);
+    i[0] = b.n_steps(m);
+    i[1] = b.n_samples(m);
+  });
+}
+
+int insem_batch_advance(void* h, int32_t n) {
+  return guard([&] { static_cast<BatchSolver*>(h)->advance(n); });
+}
+
+int insem_batch_traces(void* h, int32_t m, double* out) {
+  return guard([&] {
+    const BatchSolver& b = *static_cast<BatchSolver*>(h);
+    INSEM_REQUIRE(m >= 0 && m < b.n_models(), "no model " << m << " in the batch");
+    std::size_t o = 0;
+    for (const auto& t : b.traces(m)) {
+      std::memcpy(out + o, t.data.data(), t.data.size() * sizeof(double));
+      o += t.data.size();
+    }
+  });
+}
+
+int64_t insem_batch_footprint(void* h) {
+  try {
+    return static_cast<BatchSolver*>(h)->footprint();
+  } catch (...) {
+    return -1;
+  }
 }
 
 }  // extern "C"

@@ -10,28 +10,40 @@ void check_hip(int rc) {
   if (rc != INSEM_HIP_OK) throw Error(insem_hip_last_error());
 }
 
+ModelTime model_time(const TimeConfig& t, double lambda) {
+  INSEM_REQUIRE(lambda > 0.0, "could not estimate the stable time step (empty or rigid model?)");
+  ModelTime m;
+  m.stable_dt = 2.0 / std::sqrt(lambda);
+  if (t.dt > 0.0) {
+    INSEM_REQUIRE(t.dt <= m.stable_dt, "time step " << t.dt << " s exceeds the estimated stability limit "
+                                                   << m.stable_dt << " s");
+    m.dt = t.dt;
+  } else {
+    INSEM_REQUIRE(t.safety > 0.0 && t.safety <= 1.0, "time step safety factor must be in (0, 1]");
+    m.dt = t.safety * m.stable_dt;
+  }
+  if (t.n_steps > 0)
+    m.n_steps = t.n_steps;
+  else if (t.duration > 0.0)
+    m.n_steps = static_cast<int>(std::ceil(t.duration / m.dt - 1e-9));
+  return m;
+}
+
+double device_lambda_max(insem_hip_ctx* ctx, const SetupImage& img) {
+  const std::vector<float> start = img.power_start_vectors();
+  double lambda = 0.0;
+  check_hip(insem_hip_lambda_max(ctx, start.data(), &lambda));
+  return lambda;
+}
+
 Solver::Solver(std::unique_ptr<SetupImage> image) : img_(std::move(image)) {
   INSEM_REQUIRE(img_, "solver needs a setup");
   check_hip(insem_hip_create(&img_->view, &ctx_));
   try {
-    const std::vector<float> start = img_->power_start_vectors();
-    double lambda = 0.0;
-    check_hip(insem_hip_lambda_max(ctx_, start.data(), &lambda));
-    INSEM_REQUIRE(lambda > 0.0, "could not estimate the stable time step (empty or rigid model?)");
-    stable_dt_ = 2.0 / std::sqrt(lambda);
-    const TimeConfig& t = img_->time;
-    if (t.dt > 0.0) {
-      INSEM_REQUIRE(t.dt <= stable_dt_, "time step " << t.dt << " s exceeds the estimated stability limit "
-                                                    << stable_dt_ << " s");
-      dt_ = t.dt;
-    } else {
-      INSEM_REQUIRE(t.safety > 0.0 && t.safety <= 1.0, "time step safety factor must be in (0, 1]");
-      dt_ = t.safety * stable_dt_;
-    }
-    if (t.n_steps > 0)
-      n_steps_ = t.n_steps;
-    else if (t.duration > 0.0)
-      n_steps_ = static_cast<int>(std::ceil(t.duration / dt_ - 1e-9));
+    const ModelTime m = model_time(img_->time, device_lambda_max(ctx_, *img_));
+    stable_dt_ = m.stable_dt;
+    dt_ = m.dt;
+    n_steps_ = m.n_steps;
     const std::vector<float> amps = img_->source_amplitudes(dt_, n_steps_);
     check_hip(insem_hip_set_time(ctx_, dt_, n_steps_, amps.data()));
   } catch (...) {

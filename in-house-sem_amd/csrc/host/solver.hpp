@@ -61,4 +61,14 @@ class Solver {
 // Throws insem_amd::Error with the backend's message when rc != 0.
 void check_hip(int rc);
 
+// Time step and step count from lambda_max(M^-1 K) and the time settings
+// (Solver::Solver, solver.cpp:131-144).
+struct ModelTime {
+  double stable_dt = 0.0, dt = 0.0;
+  int n_steps = 0;
+};
+ModelTime model_time(const TimeConfig& t, double lambda);
+// lambda_max by the device power iteration on a context of the model.
+double device_lambda_max(insem_hip_ctx* ctx, const SetupImage& img);
+
 }  // namespace insem_amd

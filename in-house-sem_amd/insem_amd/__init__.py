@@ -14,6 +14,7 @@ from .core import (
     Receiver,
     SetupImage,
     Simulation,
+    BatchSolver,
     Solver,
     Source,
     SourceTimeFunction,
@@ -40,6 +41,7 @@ __all__ = [
     "Receiver",
     "SetupImage",
     "Simulation",
+    "BatchSolver",
     "Solver",
     "Source",
     "SourceTimeFunction",

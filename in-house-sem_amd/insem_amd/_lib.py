@@ -50,6 +50,8 @@ def hip():
         _declare(lib, "insem_hip_destroy", c_int, c_void)
         _declare(lib, "insem_hip_set_time", c_int, c_void, c_double, c_int, P(ctypes.c_float))
         _declare(lib, "insem_hip_lambda_max", c_int, c_void, P(ctypes.c_float), P(c_double))
+        _declare(lib, "insem_hip_lambda_start", c_int, c_void, P(ctypes.c_float))
+        _declare(lib, "insem_hip_lambda_result", c_int, c_void, P(c_double))
         _declare(lib, "insem_hip_set_state", c_int, c_void, c_int, P(ctypes.c_float), P(ctypes.c_float))
         _declare(lib, "insem_hip_get_field", c_int, c_void, c_int, c_int, P(ctypes.c_float))
         _declare(lib, "insem_hip_advance", c_int, c_void, c_int)
@@ -66,6 +68,8 @@ def hip():
         _declare(lib, "insem_hip_block_time", c_int, c_void, c_int, c_int, P(c_double), P(c_int))
         _declare(lib, "insem_hip_block_points", c_int, c_void, c_int, c_int, P(c_i64), P(c_i64))
         _declare(lib, "insem_hip_footprint", c_int, c_void, P(c_i64))
+        _declare(lib, "insem_hip_create_batch", c_int, c_void, c_void, P(c_void))
+        _declare(lib, "insem_hip_set_time_batch", c_int, c_void, P(c_double), P(c_int), P(ctypes.c_float))
         _hip = lib
     return _hip
 
@@ -128,6 +132,14 @@ def host():
         d("insem_solver_traces", c_int, c_void, P(c_double))
         d("insem_solver_n_energy", c_int, c_void)
         d("insem_solver_energy", c_int, c_void, P(c_double))
+        d("insem_batch_new", c_void, P(c_void), c_int)
+        d("insem_batch_free", None, c_void)
+        d("insem_batch_context", c_void, c_void)
+        d("insem_batch_model_image", c_void, c_void, c_int)
+        d("insem_batch_info", c_int, c_void, c_int, P(c_double), P(c_int))
+        d("insem_batch_advance", c_int, c_void, c_int)
+        d("insem_batch_traces", c_int, c_void, c_int, P(c_double))
+        d("insem_batch_footprint", c_i64, c_void)
         _host = lib
     return _host
 

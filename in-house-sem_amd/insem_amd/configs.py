@@ -97,3 +97,44 @@ def layered_coupled(nx=800, nz_fluid=80, nz_solid=(160, 160), order=4, duration=
             sim.add_receiver(f"s{i}", x, -400.0, ["displacement"])
     sim.set_time(duration=duration, n_steps=n_steps, energy_every=energy_every)
     return sim
+
+
+def cased_hole(rng, duration=5e-3, z=(-1.5, 4.5), formation_radius=4.0, dz=0.08, order=None, f_max=None,
+               n_receivers=8, receiver_z0=1.0):
+    """Axisymmetric cased-hole dipole model (azimuthal order 1) drawn from
+    `rng` (numpy Generator): borehole radius U[0.08, 0.12] m, steel casing
+    8-12 mm (vp 5900, vs 3200, rho 7850), cement 20-40 mm (vp 2800, vs 1500,
+    rho 1900), 1-4 horizontal formation beds with vp U[2000, 5000],
+    vs = vp / U[1.6, 2.2], rho U[2000, 2700]; order in {4, 5, 6}; refinement
+    to 6 points per wavelength at f_max U[5, 10] kHz; a dipole point force in
+    the fluid on the axis with a U[2, 5] kHz Ricker; pressure receivers at
+    r = 0.05 m from z = receiver_z0 up in 0.15 m steps."""
+    rb = rng.uniform(0.08, 0.12)
+    t_steel = rng.uniform(0.008, 0.012)
+    t_cem = rng.uniform(0.020, 0.040)
+    n_beds = int(rng.integers(1, 5))
+    bed_z = np.sort(rng.uniform(z[0] + 0.5, z[1] - 0.5, size=n_beds - 1))
+    beds = [(z[0], "bed0")] + [(float(b), f"bed{i + 1}") for i, b in enumerate(bed_z)]
+    form = []
+    for _ in range(n_beds):
+        vp = rng.uniform(2000.0, 5000.0)
+        form.append((vp, vp / rng.uniform(1.6, 2.2), rng.uniform(2000.0, 2700.0)))
+    order = int(rng.choice([4, 5, 6])) if order is None else order
+    f_max = rng.uniform(5e3, 1e4) if f_max is None else f_max
+    f0 = rng.uniform(2e3, 5e3)
+    bh = mesh.borehole(z, [("mud", rb), ("casing", rb + t_steel), ("cement", rb + t_steel + t_cem)],
+                       formation_radius, beds=beds, dz=dz)
+    disc = insem.Discretization(order=order, f_max=f_max, points_per_wavelength=6.0, azimuthal_order=1)
+    sim = insem.Simulation(bh.mesh, disc)
+    sim.set_material(bh.regions["mud"], insem.Material.acoustic(1500.0, 1000.0))
+    sim.set_material(bh.regions["casing"], insem.Material.elastic(5900.0, 3200.0, 7850.0))
+    sim.set_material(bh.regions["cement"], insem.Material.elastic(2800.0, 1500.0, 1900.0))
+    for (_, name), (vp, vs, rho) in zip(beds, form):
+        sim.set_material(bh.regions[name], insem.Material.elastic(vp, vs, rho))
+    for name in ("bottom", "outer", "top"):
+        sim.set_boundary(bh.boundaries[name], "absorbing")
+    sim.add_source(insem.Source.dipole(0.0, 0.0, 1.0, 0.0, insem.ricker(f0)))
+    for k in range(n_receivers):
+        sim.add_receiver(f"r{k}", 0.05, receiver_z0 + 0.15 * k, ["pressure"])
+    sim.set_time(duration=duration)
+    return sim

@@ -435,6 +435,109 @@ class Solver:
         self.close()
 
 
+# ------------------------------------------------------------------ batch
+
+
+class BatchSolver:
+    """Many independent models stepped as one union on the device (no
+    counterpart in the reference, which runs one model per Solver).  Each
+    model keeps the time step and step count its own Solver would have (its
+    own power iteration), and its traces are bitwise those of running it
+    alone.  Models must share the polynomial order, coordinate system,
+    azimuthal order, media and record cadence."""
+
+    def __init__(self, models):
+        lib = _lib.host()
+        models = list(models)
+        sims = [m for m in models if isinstance(m, Simulation)]
+        built = {}
+        if sims:
+            # host setups of independent models, several at once (the native
+            # builds release the interpreter lock)
+            from concurrent.futures import ThreadPoolExecutor
+            with ThreadPoolExecutor(max_workers=min(8, len(sims))) as pool:
+                for sim, img in zip(sims, pool.map(lambda s: s.build_setup(), sims)):
+                    built[id(sim)] = img
+        images = [built[id(m)] if isinstance(m, Simulation) else m for m in models]
+        if not images:
+            raise InsemError("a batch needs at least one model")
+        handles = (ctypes.c_void_p * len(images))(*[im._h for im in images])
+        for im in images:
+            im._owned = False  # the batch takes the images over
+        self._handle = check_ptr(lib.insem_batch_new(handles, len(images)))
+        for im in images:
+            im._h = None
+        self.models = [SetupImage(lib.insem_batch_model_image(self._handle, m), owned=False)
+                       for m in range(len(images))]
+
+    @property
+    def _h(self):
+        h = getattr(self, "_handle", None)
+        if not h:
+            raise InsemError("the batch has been closed")
+        return h
+
+    def _info(self, m):
+        d = np.zeros(2)
+        i = np.zeros(3, dtype=np.int32)
+        check(_lib.host().insem_batch_info(self._h, m, _dptr(d), _iptr(i)))
+        return d, i
+
+    @property
+    def n_models(self) -> int:
+        return len(self.models)
+
+    @property
+    def max_steps(self) -> int:
+        return int(self._info(-1)[1][1])
+
+    @property
+    def step(self) -> int:
+        return int(self._info(-1)[1][2])
+
+    def dt(self, m: int) -> float:
+        return float(self._info(m)[0][0])
+
+    def n_steps(self, m: int) -> int:
+        return int(self._info(m)[1][0])
+
+    @property
+    def context(self):
+        return _lib.host().insem_batch_context(self._h)
+
+    @property
+    def footprint(self) -> int:
+        """Device bytes held by the batch."""
+        return int(_lib.host().insem_batch_footprint(self._h))
+
+    def advance(self, n_steps: int) -> None:
+        check(_lib.host().insem_batch_advance(self._h, int(n_steps)))
+
+    def run(self) -> None:
+        self.advance(self.max_steps - self.step)
+
+    def traces(self, m: int) -> Dict[Tuple[str, str], np.ndarray]:
+        """Model m's traces, keyed and shaped as Solver.traces of that model."""
+        ns = int(self._info(m)[1][1])
+        img = self.models[m]
+        buf = np.zeros(max(sum(nc for _, _, nc in img.traces) * ns, 1))
+        check(_lib.host().insem_batch_traces(self._h, m, _dptr(buf)))
+        out, o = {}, 0
+        for name, q, nc in img.traces:
+            out[(name, q)] = buf[o:o + nc * ns].reshape(ns, nc).copy()
+            o += nc * ns
+        return out
+
+    def close(self):
+        h = getattr(self, "_handle", None)
+        if h:
+            _lib.host().insem_batch_free(h)
+        self._handle = None
+
+    def __del__(self):
+        self.close()
+
+
 # -------------------------------------------------------------- simulation
 
 

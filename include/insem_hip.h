@@ -204,6 +204,12 @@ int insem_hip_set_time(insem_hip_ctx* ctx, double dt, int32_t n_steps, const flo
  * for coupled models, solid u then fluid u.  The fields are restored.
  */
 int insem_hip_lambda_max(insem_hip_ctx* ctx, const float* start, double* lambda);
+/* The same in two halves: _start queues the iterations on the context's
+ * stream and returns (the start vectors are copied), _result waits and
+ * returns lambda.  Power iterations of several contexts then overlap on the
+ * device (a batch sets up its models this way). */
+int insem_hip_lambda_start(insem_hip_ctx* ctx, const float* start);
+int insem_hip_lambda_result(insem_hip_ctx* ctx, double* lambda);
 
 /* Initial conditions of one domain ([ncomp][n_global]); before the first step
  * only (reference: Solver::set_state, solver.cpp:764-778). */
@@ -272,6 +278,35 @@ int insem_hip_block_points(insem_hip_ctx* ctx, int32_t domain, int32_t block, in
 
 /* Device bytes held by the context. */
 int insem_hip_footprint(insem_hip_ctx* ctx, int64_t* bytes);
+
+/*
+ * Batched models (no counterpart in the reference, which runs one model per
+ * Solver): the setup is the disjoint union of n_models independent models
+ * that share the polynomial order, coordinate system, azimuthal order,
+ * media and record_every.  The union steps as one model, so each element
+ * launch and tail covers every model, while each model keeps its own time
+ * step and step count: every per-point computation uses its own model's
+ * coefficients, so a model's traces and fields are bitwise those of running
+ * it alone.
+ *   point_model[d]: model of each global point of union domain d;
+ *   receiver_model: model of each receiver.
+ * Energy rows are not kept and the power iteration is not available on a
+ * batch (each model's Δt comes from its own run of insem_hip_lambda_max or
+ * from the caller).
+ */
+typedef struct insem_hip_batch {
+  int32_t n_models;
+  const int32_t* point_model[2];
+  const int32_t* receiver_model;
+} insem_hip_batch;
+
+int insem_hip_create_batch(const insem_hip_setup* setup, const insem_hip_batch* batch, insem_hip_ctx** out);
+/* dt[m], n_steps[m] per model; amplitudes [n_sources][max_m n_steps[m] + 1],
+ * each source's values at its own model's dt.  insem_hip_advance then steps
+ * the union: a model stops at its own n_steps (its last step stores a and
+ * leaves u, v as a run of that length alone does) while longer ones go on;
+ * traces of a model beyond its n_steps / record_every + 1 samples are 0. */
+int insem_hip_set_time_batch(insem_hip_ctx* ctx, const double* dt, const int32_t* n_steps, const float* amplitudes);
 
 #ifdef __cplusplus
 }
