@@ -1436,6 +1436,25 @@ int insem_hip_synchronize(insem_hip_ctx* c) {
   return guarded(c, [&] { HIP_TRY(hipStreamSynchronize(c->stream)); });
 }
 
+int insem_hip_query(insem_hip_ctx* c, int32_t* idle) {
+  if (!idle) return set_error(INSEM_HIP_EINVAL, "null argument");
+  return guarded(c, [&] {
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e != hipSuccess && e != hipErrorNotReady) HIP_TRY(e);
+    *idle = e == hipSuccess ? 1 : 0;
+  });
+}
+
+int insem_hip_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return set_error(INSEM_HIP_EINVAL, "null argument");
+  size_t f = 0, t = 0;
+  const hipError_t e = hipMemGetInfo(&f, &t);
+  if (e != hipSuccess) return set_error(INSEM_HIP_EDEVICE, "hipMemGetInfo failed: %s", hipGetErrorString(e));
+  *free_bytes = static_cast<int64_t>(f);
+  *total_bytes = static_cast<int64_t>(t);
+  return INSEM_HIP_OK;
+}
+
 int insem_hip_set_timing(insem_hip_ctx* c, int32_t enabled) {
   return guarded(c, [&] {
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -138,3 +138,36 @@ def cased_hole(rng, duration=5e-3, z=(-1.5, 4.5), formation_radius=4.0, dz=0.08,
         sim.add_receiver(f"r{k}", 0.05, receiver_z0 + 0.15 * k, ["pressure"])
     sim.set_time(duration=duration)
     return sim
+
+
+def open_hole(rng, duration=3e-3, z=(-1.5, 4.5), formation_radius=4.0, dz=0.08, order=None, f_max=None,
+              n_receivers=8, receiver_z0=1.0):
+    """Axisymmetric open-hole monopole model (azimuthal order 0) drawn from
+    `rng`: mud column of radius U[0.08, 0.12] m (vp 1500, rho 1000), 1-4
+    formation beds with vp U[2000, 5000], vs = vp / U[1.6, 2.2], rho
+    U[2000, 2700]; order in {4, 5, 6}; refinement to 6 points per
+    wavelength at f_max U[5, 10] kHz; an on-axis pressure monopole with a
+    Ricker of f_max / 2.5; pressure receivers at r = 0.05 m from z =
+    receiver_z0 up in 0.15 m steps."""
+    rb = rng.uniform(0.08, 0.12)
+    n_beds = int(rng.integers(1, 5))
+    bed_z = np.sort(rng.uniform(z[0] + 0.5, z[1] - 0.5, size=n_beds - 1))
+    beds = [(z[0], "bed0")] + [(float(b), f"bed{i + 1}") for i, b in enumerate(bed_z)]
+    form = []
+    for _ in range(n_beds):
+        vp = rng.uniform(2000.0, 5000.0)
+        form.append((vp, vp / rng.uniform(1.6, 2.2), rng.uniform(2000.0, 2700.0)))
+    order = int(rng.choice([4, 5, 6])) if order is None else order
+    f_max = rng.uniform(5e3, 1e4) if f_max is None else f_max
+    bh = mesh.borehole(z, [("mud", rb)], formation_radius, beds=beds, dz=dz)
+    sim = insem.Simulation(bh.mesh, insem.Discretization(order=order, f_max=f_max, points_per_wavelength=6.0))
+    sim.set_material(bh.regions["mud"], insem.Material.acoustic(1500.0, 1000.0))
+    for (_, name), (vp, vs, rho) in zip(beds, form):
+        sim.set_material(bh.regions[name], insem.Material.elastic(vp, vs, rho))
+    for name in ("bottom", "outer", "top"):
+        sim.set_boundary(bh.boundaries[name], "absorbing")
+    sim.add_source(insem.Source.monopole(0.0, 0.0, insem.ricker(f_max / 2.5)))
+    for k in range(n_receivers):
+        sim.add_receiver(f"r{k}", 0.05, receiver_z0 + 0.15 * k, ["pressure"])
+    sim.set_time(duration=duration)
+    return sim

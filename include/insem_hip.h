@@ -254,6 +254,11 @@ int insem_hip_block_forces(insem_hip_ctx* ctx, int32_t domain, int32_t block, co
 
 /* Blocks until all work of the context has finished. */
 int insem_hip_synchronize(insem_hip_ctx* ctx);
+/* *idle = 1 when all queued work of the context has finished, else 0; does
+ * not block (a request loop polls it to keep several contexts busy). */
+int insem_hip_query(insem_hip_ctx* ctx, int32_t* idle);
+/* Free and total bytes of the current device (admission control). */
+int insem_hip_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 
 /*
  * Kernel timing with HIP events on the context's stream.  When enabled,
