@@ -17,7 +17,9 @@ launch measured with rocprofv3 counters (read from the committed summary
 under profiles/), the CPU oracle timed on the host cores on a bounded
 sample (about 20 s) of the same workload, and (under "batched") the solves/s
 and latencies of cased-hole dipole models run as batches against the same
-models run one after another on the plain path.
+models run one after another on the plain path, and (under "served") the
+throughput and p50/p99 latency of a Poisson stream of mixed requests served
+with continuous batching against the same stream served one at a time.
 
 `--calibrate-copy` instead runs a plain device-to-device copy of known size
 (the reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters).
@@ -173,6 +175,47 @@ def batched_leg(n_models, duration, seed=2026):
             "batches": len(orders)}
 
 
+def served_leg(n_requests, rate, seed=7):
+    """A Poisson stream of mixed requests (60 % cased-hole dipole, 30 %
+    open-hole monopole at 3-8 ms; 10 % 2D layered coupled at 1-4 M nodes for
+    0.5-1 s) served by the continuous-batching loop, then the same stream
+    served one request at a time on the plain path (one slot, no batching).
+    Latency = arrival to traces (and STC for boreholes) ready."""
+    import numpy as np
+
+    from insem_amd import serve
+    from insem_amd.configs import cased_hole, layered_coupled, open_hole
+
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    kinds = rng.choice(["cased", "open", "layered"], p=[0.6, 0.3, 0.1], size=n_requests)
+    seeds = rng.integers(0, 2**31, size=n_requests)
+
+    def build(kind, s):
+        r = np.random.default_rng(int(s))
+        if kind == "cased":
+            return cased_hole(r, duration=r.uniform(3e-3, 8e-3))
+        if kind == "open":
+            return open_hole(r, duration=r.uniform(3e-3, 8e-3))
+        nx = int(r.integers(250, 501)) * 2  # 1-4 M nodes at order 4
+        nzf, nzs = nx // 10, (nx * 2) // 10
+        return layered_coupled(nx=nx, nz_fluid=nzf, nz_solid=(nzs, nzs), duration=r.uniform(0.5, 1.0),
+                               n_receivers=101)
+
+    stc = dict(slowness=np.linspace(100e-6, 1500e-6, 141), window=5e-4, times=np.linspace(0.0, 2.5e-3, 51))
+    out = {}
+    for name, slots, batch in (("served", 4, 8), ("one_at_a_time", 1, 1)):
+        stream = [(float(t), build(k, s)) for t, k, s in zip(arrivals, kinds, seeds)]
+        srv = serve.Server(max_slots=slots, max_batch=batch, stc=stc)
+        t0 = time.perf_counter()
+        reqs = srv.serve(stream)
+        out[name] = serve.summary(reqs, time.perf_counter() - t0)
+    out["sample"] = (f"{n_requests} requests, Poisson arrivals at {rate:g}/s (seed {seed}): "
+                     f"{int((kinds == 'cased').sum())} cased-hole, {int((kinds == 'open').sum())} open-hole, "
+                     f"{int((kinds == 'layered').sum())} layered; served: 4 slots, batches of up to 8")
+    return out
+
+
 def calibrate_copy(n_bytes=1 << 30, reps=5):
     """A plain copy of n_bytes between two device buffers, reps times."""
     import torch
@@ -195,6 +238,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="run only a 1 GiB device copy (counter scale)")
     ap.add_argument("--batch-models", type=int, default=24, help="models of the batched leg (0: skip it)")
     ap.add_argument("--batch-duration", type=float, default=5e-3, help="simulated time of each batched model (s)")
+    ap.add_argument("--serve-requests", type=int, default=16, help="requests of the served leg (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the served leg (1/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every run of this build uses a single MI355X "
@@ -290,6 +335,8 @@ def main():
     solver.close()  # free the model's device memory before the other legs
     if args.batch_models > 0:
         result["batched"] = batched_leg(args.batch_models, args.batch_duration)
+    if args.serve_requests > 0:
+        result["served"] = served_leg(args.serve_requests, args.serve_rate)
     result["cpu_baseline"] = None if args.no_cpu else cpu_baseline(sim, dt)
     print(json.dumps(result), flush=True)
 

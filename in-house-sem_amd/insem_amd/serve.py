@@ -11,6 +11,11 @@ only for a slot, not for the others to finish.  Every slot is kept busy by
 queueing its next chunk of steps as soon as its stream is idle; a finished
 batch returns each request's traces (bitwise those of the request run
 alone) and, for borehole arrays, their slowness-time coherence.
+
+Host work that does not touch the device (building a request's setup on
+arrival, the coherence of finished traces) runs on worker threads, so the
+loop keeps feeding the device meanwhile; every device call stays on the
+loop's thread.
 """
 
 from __future__ import annotations
@@ -18,6 +23,7 @@ from __future__ import annotations
 import ctypes
 import time
 from collections import deque
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence, Tuple
 
@@ -95,7 +101,7 @@ class Server:
     post.stc for each borehole request's first quantity over its receivers
     (slowness grid, window, ...; offsets are the receivers' z), or None."""
 
-    def __init__(self, max_slots: int = 4, max_batch: int = 16, chunk: int = 256,
+    def __init__(self, max_slots: int = 4, max_batch: int = 16, chunk: int = 1024,
                  hbm_budget: Optional[int] = None, stc: Optional[dict] = None):
         self.max_slots = max_slots
         self.max_batch = max_batch
@@ -126,61 +132,87 @@ class Server:
             r.dt, r.n_steps = b.dt(m), b.n_steps(m)
         return _Slot(b, group, need)
 
-    def _finish(self, slot: _Slot, t0: float) -> None:
+    def _coherence(self, r: Request, step: float) -> None:
+        q = r.sim.receivers[0].quantities[0]
+        recs = [rec for rec in r.sim.receivers if (rec.name, q) in r.traces]
+        x = np.array([r.traces[(rec.name, q)][:, 0] for rec in recs])
+        z = np.array([rec.position[1] for rec in recs])
+        r.stc = post.stc(x, z, step, **self.stc)
+
+    def _finish(self, slot: _Slot, pool: ThreadPoolExecutor, t0: float) -> List[Future]:
+        """Reads the traces of a finished batch; the coherence of borehole
+        arrays goes to the workers.  Returns their futures."""
         b = slot.batch
+        jobs = []
         for m, r in enumerate(slot.requests):
             r.traces = b.traces(m)
             if self.stc is not None and b.models[m].axisymmetric and len(r.sim.receivers) > 1:
-                q = r.sim.receivers[0].quantities[0]
-                recs = [rec for rec in r.sim.receivers if (rec.name, q) in r.traces]
-                x = np.array([r.traces[(rec.name, q)][:, 0] for rec in recs])
-                z = np.array([rec.position[1] for rec in recs])
-                r.stc = post.stc(x, z, b.models[m].time.record_every * r.dt, **self.stc)
-            r.done = time.perf_counter() - t0
+                step = b.models[m].time.record_every * r.dt
+
+                def job(r=r, step=step):
+                    self._coherence(r, step)
+                    r.done = time.perf_counter() - t0
+
+                jobs.append(pool.submit(job))
+            else:
+                r.done = time.perf_counter() - t0
         b.close()
+        return jobs
+
+    @staticmethod
+    def _prepare(r: Request) -> Request:
+        r.image = r.sim.build_setup()
+        r.key = batch_key(r.image)
+        r.footprint = footprint_estimate(r.image)
+        return r
 
     def serve(self, arrivals: Sequence[Tuple[float, Simulation]]) -> List[Request]:
         """Runs the stream (arrival times in s from now) to completion; returns
         the requests in arrival order with their timings and results."""
         reqs = [Request(i, sim, float(t)) for i, (t, sim) in enumerate(sorted(arrivals, key=lambda a: a[0]))]
         pending = deque(reqs)
+        building: deque = deque()   # futures of setups in arrival order
         queue: deque = deque()
         slots: List[_Slot] = []
+        jobs: List[Future] = []
         used = 0
         t0 = time.perf_counter()
-        while pending or queue or slots:
-            now = time.perf_counter() - t0
-            while pending and pending[0].arrival <= now:
-                r = pending.popleft()
-                r.image = r.sim.build_setup()
-                r.key = batch_key(r.image)
-                r.footprint = footprint_estimate(r.image)
-                if r.footprint > self.hbm_budget:
-                    raise MemoryError(f"request {r.rid} needs about {r.footprint} device bytes, "
-                                      f"more than the budget of {self.hbm_budget}")
-                queue.append(r)
-            while queue and len(slots) < self.max_slots:
-                slot = self._admit(queue, time.perf_counter() - t0, used)
-                if slot is None:
-                    break
-                slots.append(slot)
-                used += slot.footprint
-            busy = False
-            for slot in list(slots):
-                b = slot.batch
-                if b.step >= b.max_steps:
-                    if slot.idle():
-                        self._finish(slot, t0)
-                        slots.remove(slot)
-                        used -= slot.footprint
-                elif slot.idle():
-                    b.advance(self.chunk)
-                    busy = True
-            if not busy:
-                if not slots and not queue and pending:
-                    time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
-                else:
-                    time.sleep(2e-4)
+        with ThreadPoolExecutor(max_workers=4) as pool:
+            while pending or building or queue or slots or jobs:
+                now = time.perf_counter() - t0
+                while pending and pending[0].arrival <= now:
+                    building.append(pool.submit(self._prepare, pending.popleft()))
+                while building and building[0].done():
+                    r = building.popleft().result()
+                    if r.footprint > self.hbm_budget:
+                        raise MemoryError(f"request {r.rid} needs about {r.footprint} device bytes, "
+                                          f"more than the budget of {self.hbm_budget}")
+                    queue.append(r)
+                while queue and len(slots) < self.max_slots:
+                    slot = self._admit(queue, time.perf_counter() - t0, used)
+                    if slot is None:
+                        break
+                    slots.append(slot)
+                    used += slot.footprint
+                busy = False
+                for slot in list(slots):
+                    b = slot.batch
+                    if b.step >= b.max_steps:
+                        if slot.idle():
+                            jobs += self._finish(slot, pool, t0)
+                            slots.remove(slot)
+                            used -= slot.footprint
+                    elif slot.idle():
+                        b.advance(self.chunk)
+                        busy = True
+                for j in [j for j in jobs if j.done()]:
+                    j.result()
+                    jobs.remove(j)
+                if not busy:
+                    if not slots and not queue and not building and not jobs and pending:
+                        time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+                    else:
+                        time.sleep(2e-4)
         return reqs
 
 
