@@ -604,6 +604,16 @@ __device__ __forceinline__ void write_energy(double ep, double ek, double* red, 
 // One block per launch.  In a time step the first launch also carries the
 // previous step's receiver tasks in its first n_rec work-groups (they read
 // the side buffers before this step's tails rewrite them).
+// Waves per SIMD the gathering force kernel is compiled for: the eight
+// that the hardware holds wherever the registers fit without spilling (the
+// kernel waits on two dependent global round trips per work-group, so
+// resident work-groups are what hides them).
+constexpr int force_waves(int ng, int phys) {
+  if (phys == kElasticAxi1) return ng <= 3 ? 8 : ng <= 7 ? 5 : 4;
+  if (phys == kElasticAxi0) return ng <= 5 ? 8 : ng <= 8 ? 6 : 5;
+  return 8;
+}
+
 template <int NG, int PHYS, bool RECT>
 struct ForceLds {
   static constexpr size_t tiles = sizeof(Tiles<NG, PHYS>);
@@ -612,7 +622,7 @@ struct ForceLds {
 };
 
 template <int NG, int PHYS, bool RECT>
-__global__ __launch_bounds__(kWG) void k_forces(ForceArgs A, StepParams P, ReceiverDev R, int n_rec,
+__global__ __launch_bounds__(kWG, force_waves(NG, PHYS)) void k_forces(ForceArgs A, StepParams P, ReceiverDev R, int n_rec,
                                                 double half_dt) {
   __shared__ __align__(16) unsigned char smem[ForceLds<NG, PHYS, RECT>::bytes];
   __shared__ double red[kWG];

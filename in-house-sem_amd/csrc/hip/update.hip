@@ -35,20 +35,41 @@ __device__ double block_sum(double v, double* lds) {
   return r;
 }
 
+// Assembled force of point g from f_local through its CSR row, summed in
+// slot order.  The row is read in chunks of kChunk slots whose addresses,
+// then values, are all loaded before the chunk's adds.
 template <int C, bool W>
 __device__ __forceinline__ void gather_sum(const DomDev& d, int64_t g, float* f) {
+  constexpr int kChunk = 4;
 #pragma unroll
   for (int c = 0; c < C; ++c) f[c] = 0.0f;
   const int32_t s0 = d.t_off[g], s1 = d.t_off[g + 1];
-  for (int32_t s = s0; s < s1; ++s) {
-    const float* p = d.fl + d.t_addr[s];
-    if constexpr (W) {
-      const float w = d.t_w[s];
+  for (int32_t s = s0; s < s1; s += kChunk) {
+    int64_t at[kChunk];
+    float w[kChunk];
 #pragma unroll
-      for (int c = 0; c < C; ++c) f[c] += w * p[c * d.npe];
-    } else {
+    for (int q = 0; q < kChunk; ++q) {
+      const bool in = s + q < s1;
+      at[q] = in ? d.t_addr[s + q] : 0;
+      if constexpr (W) w[q] = in ? d.t_w[s + q] : 0.0f;
+    }
+    float p[kChunk][C];
 #pragma unroll
-      for (int c = 0; c < C; ++c) f[c] += p[c * d.npe];
+    for (int q = 0; q < kChunk; ++q) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) p[q][c] = s + q < s1 ? d.fl[at[q] + c * d.npe] : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) {
+      if (s + q < s1) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          if constexpr (W)
+            f[c] += w[q] * p[q][c];
+          else
+            f[c] += p[q][c];
+        }
+      }
     }
   }
 }
@@ -131,6 +152,22 @@ __device__ __forceinline__ void regular_body(const DomDev& d, int64_t first, con
   update_point<C>(d, ps, corr, energy, g, f, ep, ek);
 }
 
+// Sums of the K contributions per component of a grouped point, from zero
+// in rank order; all C * K loads are issued before the first add.
+template <int C, int K>
+__device__ __forceinline__ void group_sum(const float* src, int64_t n, float* f) {
+  float x[C * K];
+#pragma unroll
+  for (int q = 0; q < C * K; ++q) x[q] = src[q * n];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    float s = 0.0f;
+#pragma unroll
+    for (int r = 0; r < K; ++r) s += x[c * K + r];
+    f[c] = s;
+  }
+}
+
 // Grouped points: the contributions of a point lie in rank order at stride
 // size[q] in the group layout; the sum starts from zero in slot order as the
 // assembly's does.
@@ -146,11 +183,14 @@ __device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc
   group_locate(d.grp, g, &off, &n, &cnt);
   const float* src = d.contrib + off;
   float f[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    float s = 0.0f;
-    for (int r = 0; r < cnt; ++r) s += src[(c * cnt + r) * n];
-    f[c] = s;
+  switch (cnt) {
+    case 2: group_sum<C, 2>(src, n, f); break;
+    case 3: group_sum<C, 3>(src, n, f); break;
+    case 4: group_sum<C, 4>(src, n, f); break;
+    case 5: group_sum<C, 5>(src, n, f); break;
+    case 6: group_sum<C, 6>(src, n, f); break;
+    case 7: group_sum<C, 7>(src, n, f); break;
+    default: group_sum<C, 8>(src, n, f); break;
   }
   update_point<C>(d, ps, corr, energy, g, f, ep, ek);
 }
@@ -277,9 +317,14 @@ __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& 
 // the special points (first, so that their dependent loads start early),
 // the next n_grp_blocks the grouped points, the rest the regular points
 // [reg_first, n_reg) through the CSR.  Energy partials: [block][2].
+// Waves per SIMD of the tail: all eight the hardware holds, for one and two
+// components (the kernel is bound by the latency of its gathers).
+constexpr int tail_waves(int c) { return c <= 2 ? 8 : 5; }
+
 template <int C, bool W>
-__global__ __launch_bounds__(kTB) void k_tail(DomDev d, SpecialDev sp, StepCoef sc, int corr, int pred, StepClock clk,
-                                              double* partial, int n_sp_blocks, int n_grp_blocks, int64_t reg_first) {
+__global__ __launch_bounds__(kTB, tail_waves(C)) void k_tail(DomDev d, SpecialDev sp, StepCoef sc, int corr, int pred,
+                                                             StepClock clk, double* partial, int n_sp_blocks,
+                                                             int n_grp_blocks, int64_t reg_first) {
   __shared__ double lds[kTB];
   const int step = clock_step(clk);
   const bool energy = clock_energy(clk, step);
