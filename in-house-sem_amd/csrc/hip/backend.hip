@@ -271,6 +271,14 @@ int validate_setup(const insem_hip_setup* s) {
     if (seen[d.medium]++) return set_error(INSEM_HIP_EINVAL, "two domains of the same medium");
     const int want = d.medium == INSEM_MEDIUM_ELASTIC ? (d.azimuthal_order == 1 ? 3 : 2) : 1;
     if (d.ncomp != want) return set_error(INSEM_HIP_EINVAL, "domain %d has %d components, expected %d", di, d.ncomp, want);
+    // The kernels address fields and element arrays with 32-bit offsets.
+    const int64_t npe = static_cast<int64_t>(s->ngll) * s->ngll;
+    if (static_cast<int64_t>(d.ncomp) * d.n_global >= kMaxIndex32 || d.ncomp * npe * d.n_elements >= kMaxIndex32)
+      return set_error(INSEM_HIP_EINVAL,
+                       "domain %d is too large for one context (%lld points, %lld elements; at most %lld "
+                       "values per field and per element array)",
+                       di, static_cast<long long>(d.n_global), static_cast<long long>(d.n_elements),
+                       static_cast<long long>(kMaxIndex32 - 1));
     int64_t covered = 0;
     for (int b = 0; b < d.n_blocks; ++b) {
       const insem_hip_block& B = d.blocks[b];

@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 namespace insem_hip {
 
@@ -36,8 +37,22 @@ struct PF {
   int sk = 0;
 };
 
-__device__ __forceinline__ float ld(const PF& f, int b, int k) {
-  return f.p[static_cast<size_t>(b) * f.sb + static_cast<size_t>(k) * f.sk];
+// Element i of an array that the kernels index with 32-bit offsets: a
+// uniform base plus a 32-bit byte offset, which the load and store
+// instructions take directly (no 64-bit address arithmetic per access).
+// Every such array holds fewer than kMaxIndex32 elements; the host checks
+// this when it builds a context.
+constexpr int64_t kMaxIndex32 = int64_t{1} << 30;
+
+template <class T>
+__device__ __forceinline__ T* at32(T* p, uint32_t i) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T*>(reinterpret_cast<B*>(p) + i * static_cast<uint32_t>(sizeof(T)));
+}
+
+// Value of a per-point quantity at slot k of element b; slot = b * npe + k.
+__device__ __forceinline__ float ld(const PF& f, uint32_t b, uint32_t slot) {
+  return *at32(f.p, f.sk ? slot : b);
 }
 
 // Step number of a launch: *base (the step before the chunk, on the device)

@@ -120,23 +120,23 @@ struct Ops {
 };
 
 template <int PHYS, bool RECT>
-__device__ __forceinline__ void load_ops(const ForceArgs& A, int b, int k, Ops& o) {
+__device__ __forceinline__ void load_ops(const ForceArgs& A, uint32_t b, uint32_t slot, Ops& o) {
   constexpr int C = Shape<PHYS>::C;
   constexpr bool AXI = Shape<PHYS>::AXI;
-  o.q0 = ld(A.m[0], b, k);
-  o.q3 = ld(A.m[3], b, k);
-  if constexpr (!AXI) o.q4 = ld(A.m[4], b, k);
+  o.q0 = ld(A.m[0], b, slot);
+  o.q3 = ld(A.m[3], b, slot);
+  if constexpr (!AXI) o.q4 = ld(A.m[4], b, slot);
   if constexpr (!RECT) {
-    o.q1 = ld(A.m[1], b, k);
-    o.q2 = ld(A.m[2], b, k);
+    o.q1 = ld(A.m[1], b, slot);
+    o.q2 = ld(A.m[2], b, slot);
   }
-  o.k0 = ld(A.c0, b, k);
-  if constexpr (C >= 2) o.k1 = ld(A.c1, b, k);
+  o.k0 = ld(A.c0, b, slot);
+  if constexpr (C >= 2) o.k1 = ld(A.c1, b, slot);
   if constexpr (AXI) {
-    o.qw = ld(A.wr, b, k);
-    o.qr = ld(A.r, b, k);
-    if constexpr (PHYS != kAcousticAxi0) o.qh = ld(A.hoop, b, k);
-    o.axial = A.axial[b] != 0;
+    o.qw = ld(A.wr, b, slot);
+    o.qr = ld(A.r, b, slot);
+    if constexpr (PHYS != kAcousticAxi0) o.qh = ld(A.hoop, b, slot);
+    o.axial = *at32(A.axial, b) != 0;
   }
 }
 
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(kWG) void k_forces_local(ForceArgs A) {
   stage_matrices(A, T);
   Ops o;
   if (act) {
-    load_ops<PHYS, RECT>(A, b, k, o);
+    load_ops<PHYS, RECT>(A, b, b * NPE + k, o);
     const size_t e = static_cast<size_t>(A.elem[b]);
 #pragma unroll
     for (int c = 0; c < C; ++c) T.u[le * C * NPE + c * NPE + k] = A.ul[(e * C + c) * NPE + k];
@@ -460,9 +460,10 @@ __device__ __forceinline__ void stage_point(const ForceArgs& A, const StepParams
   if (s.act && P.fuse && A.models) s.act = step <= A.models[A.emodel[b]].n_steps;
   s.b = b;
   if (s.act) {
-    s.m = A.l2g[static_cast<size_t>(b) * NPE + k];
-    load_ops<PHYS, RECT>(A, b, k, s.o);
-    s.rank = A.rank[static_cast<size_t>(b) * NPE + k];
+    const uint32_t slot = b * NPE + k;
+    s.m = *at32(A.l2g, slot);
+    load_ops<PHYS, RECT>(A, b, slot, s.o);
+    s.rank = *at32(A.rank, slot);
   }
 }
 
@@ -473,14 +474,15 @@ template <int C>
 __device__ __forceinline__ void stage_gather(const ForceArgs& A, int fuse, Slot<C>& s) {
   if (!s.act) return;
   const int32_t m = s.m;
-  const int64_t mg = m >= 0 ? m : 0;
+  const uint32_t ng = static_cast<uint32_t>(A.ng);
+  const uint32_t mg = m >= 0 ? m : 0;
 #pragma unroll
-  for (int c = 0; c < C; ++c) s.uu[c] = A.u[c * A.ng + mg];
+  for (int c = 0; c < C; ++c) s.uu[c] = *at32(A.u, c * ng + mg);
   s.owned = fuse && m >= 0 && m < A.n_int;
-  const int64_t mo = s.owned ? m : 0;
+  const uint32_t mo = s.owned ? m : 0;
 #pragma unroll
-  for (int c = 0; c < C; ++c) s.pv[c] = A.v[c * A.ng + mo];
-  s.pim = A.im[mo];
+  for (int c = 0; c < C; ++c) s.pv[c] = *at32(A.v, c * ng + mo);
+  s.pim = *at32(A.im, mo);
   if (m < 0) {
     const int32_t r = -1 - m;
 #pragma unroll
@@ -502,7 +504,8 @@ __device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepPara
     // assembly's sum of one term); the update follows the tail's
     // expressions in the same order (update.hip), with the model's own
     // coefficients on a batch (whose last step predicts nothing).
-    const int64_t g = s.m;
+    const uint32_t g = s.m;
+    const uint32_t ng = static_cast<uint32_t>(A.ng);
     float rdt = P.rdt, hdt = P.hdt, hdt2 = P.hdt2;
     double cc = P.cc;
     int pred = P.pred;
@@ -516,7 +519,7 @@ __device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepPara
     }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      const int64_t o = c * A.ng + g;
+      const uint32_t o = c * ng + g;
       float u = U[c * NPE + k], v = s.pv[c];
       const float f = 0.0f + fo[c];
       const float a = f * s.pim;
@@ -524,16 +527,16 @@ __device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepPara
       if (P.corr) v = v + hdt * a;
       if (energy) {
         const double vd = v, ad = a;
-        ek += static_cast<double>(A.mass[g]) * (0.5 * vd * vd - cc * ad * ad);
+        ek += static_cast<double>(*at32(A.mass, g)) * (0.5 * vd * vd - cc * ad * ad);
       }
       if (pred) {
         u = u + (rdt * v + hdt2 * a);
         v = v + hdt * a;
-        A.uw[o] = u;
+        *at32(A.uw, o) = u;
       } else {
-        A.a[o] = a;
+        *at32(A.a, o) = a;
       }
-      A.v[o] = v;
+      *at32(A.v, o) = v;
     }
     return;
   }
@@ -541,14 +544,14 @@ __device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepPara
     int64_t off, n;
     int cnt;
     group_locate(A.grp, s.m, &off, &n, &cnt);
-    float* dst = A.contrib + off + s.rank * n;
+    const uint32_t q = static_cast<uint32_t>(off) + s.rank * static_cast<uint32_t>(n);
 #pragma unroll
-    for (int c = 0; c < C; ++c) dst[c * cnt * n] = fo[c];
+    for (int c = 0; c < C; ++c) *at32(A.contrib, q + c * cnt * static_cast<uint32_t>(n)) = fo[c];
     return;
   }
-  float* out = A.fl + (static_cast<size_t>(A.out_pos0) + s.b) * C * NPE;
+  const uint32_t out = (static_cast<uint32_t>(A.out_pos0) + s.b) * (C * NPE) + k;
 #pragma unroll
-  for (int c = 0; c < C; ++c) out[c * NPE + k] = fo[c];
+  for (int c = 0; c < C; ++c) *at32(A.fl, out + c * NPE) = fo[c];
 }
 
 // One batch of a block from the global field.

@@ -39,25 +39,25 @@ __device__ double block_sum(double v, double* lds) {
 // slot order.  The row is read in chunks of kChunk slots whose addresses,
 // then values, are all loaded before the chunk's adds.
 template <int C, bool W>
-__device__ __forceinline__ void gather_sum(const DomDev& d, int64_t g, float* f) {
+__device__ __forceinline__ void gather_sum(const DomDev& d, uint32_t g, float* f) {
   constexpr int kChunk = 4;
 #pragma unroll
   for (int c = 0; c < C; ++c) f[c] = 0.0f;
-  const int32_t s0 = d.t_off[g], s1 = d.t_off[g + 1];
+  const int32_t s0 = *at32(d.t_off, g), s1 = *at32(d.t_off, g + 1);
   for (int32_t s = s0; s < s1; s += kChunk) {
-    int64_t at[kChunk];
+    uint32_t at[kChunk];
     float w[kChunk];
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
       const bool in = s + q < s1;
-      at[q] = in ? d.t_addr[s + q] : 0;
-      if constexpr (W) w[q] = in ? d.t_w[s + q] : 0.0f;
+      at[q] = in ? *at32(d.t_addr, s + q) : 0;
+      if constexpr (W) w[q] = in ? *at32(d.t_w, s + q) : 0.0f;
     }
     float p[kChunk][C];
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
 #pragma unroll
-      for (int c = 0; c < C; ++c) p[q][c] = s + q < s1 ? d.fl[at[q] + c * d.npe] : 0.0f;
+      for (int c = 0; c < C; ++c) p[q][c] = s + q < s1 ? *at32(d.fl, at[q] + c * d.npe) : 0.0f;
     }
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
@@ -87,7 +87,7 @@ struct PointStep {
 __device__ __forceinline__ PointStep point_step(const DomDev& d, const StepCoef& sc, int pred, int step, int64_t g) {
   PointStep p{sc.rdt, sc.hdt, sc.hdt2, sc.cc, pred, true};
   if (d.models) {
-    const ModelCoef& m = d.models[d.pmodel[g]];
+    const ModelCoef& m = d.models[*at32(d.pmodel, static_cast<uint32_t>(g))];
     p.rdt = m.rdt;
     p.hdt = m.hdt;
     p.hdt2 = m.hdt2;
@@ -103,28 +103,29 @@ __device__ __forceinline__ PointStep point_step(const DomDev& d, const StepCoef&
 template <int C>
 __device__ __forceinline__ void update_point(const DomDev& d, const PointStep& ps, int corr, bool energy, int64_t g,
                                              const float* f, double& ep, double& ek) {
-  const float im = d.im[g];
+  const uint32_t ng = static_cast<uint32_t>(d.ng), gi = static_cast<uint32_t>(g);
+  const float im = *at32(d.im, gi);
 #pragma unroll
   for (int c = 0; c < C; ++c) {
-    const int64_t o = c * d.ng + g;
-    float u = d.u[o], v = d.v[o];
+    const uint32_t o = c * ng + gi;
+    float u = *at32(d.u, o), v = *at32(d.v, o);
     const float a = f[c] * im;
     if (energy) ep += static_cast<double>(u) * static_cast<double>(f[c]);
     if (corr) v = v + ps.hdt * a;
     if (energy) {
       const double vd = v, ad = a;
-      ek += static_cast<double>(d.mass[g]) * (0.5 * vd * vd - ps.cc * ad * ad);
+      ek += static_cast<double>(*at32(d.mass, gi)) * (0.5 * vd * vd - ps.cc * ad * ad);
     }
     if (ps.pred) {
       // The next step recomputes a at regular points before anything
       // reads it, so it is only stored when the step ends the call.
       u = u + (ps.rdt * v + ps.hdt2 * a);
       v = v + ps.hdt * a;
-      d.u[o] = u;
+      *at32(d.u, o) = u;
     } else {
-      d.a[o] = a;
+      *at32(d.a, o) = a;
     }
-    d.v[o] = v;
+    *at32(d.v, o) = v;
   }
 }
 
@@ -155,10 +156,10 @@ __device__ __forceinline__ void regular_body(const DomDev& d, int64_t first, con
 // Sums of the K contributions per component of a grouped point, from zero
 // in rank order; all C * K loads are issued before the first add.
 template <int C, int K>
-__device__ __forceinline__ void group_sum(const float* src, int64_t n, float* f) {
+__device__ __forceinline__ void group_sum(const float* contrib, uint32_t off, uint32_t n, float* f) {
   float x[C * K];
 #pragma unroll
-  for (int q = 0; q < C * K; ++q) x[q] = src[q * n];
+  for (int q = 0; q < C * K; ++q) x[q] = *at32(contrib, off + q * n);
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     float s = 0.0f;
@@ -181,16 +182,16 @@ __device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc
   int64_t off, n;
   int cnt;
   group_locate(d.grp, g, &off, &n, &cnt);
-  const float* src = d.contrib + off;
+  const uint32_t o = static_cast<uint32_t>(off), z = static_cast<uint32_t>(n);
   float f[C];
   switch (cnt) {
-    case 2: group_sum<C, 2>(src, n, f); break;
-    case 3: group_sum<C, 3>(src, n, f); break;
-    case 4: group_sum<C, 4>(src, n, f); break;
-    case 5: group_sum<C, 5>(src, n, f); break;
-    case 6: group_sum<C, 6>(src, n, f); break;
-    case 7: group_sum<C, 7>(src, n, f); break;
-    default: group_sum<C, 8>(src, n, f); break;
+    case 2: group_sum<C, 2>(d.contrib, o, z, f); break;
+    case 3: group_sum<C, 3>(d.contrib, o, z, f); break;
+    case 4: group_sum<C, 4>(d.contrib, o, z, f); break;
+    case 5: group_sum<C, 5>(d.contrib, o, z, f); break;
+    case 6: group_sum<C, 6>(d.contrib, o, z, f); break;
+    case 7: group_sum<C, 7>(d.contrib, o, z, f); break;
+    default: group_sum<C, 8>(d.contrib, o, z, f); break;
   }
   update_point<C>(d, ps, corr, energy, g, f, ep, ek);
 }
