@@ -133,6 +133,7 @@ struct insem_hip_ctx {
   int chunk = 32;
   int use_graphs = 1;
   int fuse = 1;  // owned points updated inside the force launches
+  int pair = 1;  // a step's elastic and acoustic force work in one launch
   // batch of independent models (0: a single model)
   int n_models = 0;
   ModelCoef* d_models = nullptr;
@@ -865,12 +866,36 @@ void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, 
   hipStream_t s0 = c->stream;
   int row = 0, nb = 0;
   bool rec_taken = !rec_prev || c->rec.n_tasks == 0;
+  // The first elastic and the first acoustic block share one launch where a
+  // paired kernel exists for their physics.
+  BlockRT* pa = nullptr;
+  BlockRT* pb = nullptr;
+  if (c->fuse && c->pair && c->solid >= 0 && c->fluid >= 0 && !c->doms[c->solid].blocks.empty() &&
+      !c->doms[c->fluid].blocks.empty()) {
+    pa = &c->doms[c->solid].blocks[0];
+    pb = &c->doms[c->fluid].blocks[0];
+    if (!forces_pair_supported(pa->phys, pa->rect, pb->phys, pb->rect)) pa = pb = nullptr;
+  }
+  if (pa) {
+    const StepParams p = fused_params(sc, corr, pred, clk, c->partial + 2 * row);
+    const bool with_rec = !rec_taken;
+    c->cur_tag = c->solid * 64;
+    hipEvent_t e0 = nullptr;
+    timed_begin(c, &e0);
+    int grid = 0;
+    HIP_TRY(launch_forces_pair(c->ngll, pa->phys, pa->rect, pb->phys, pb->rect, pa->args, pb->args, p, s0, &grid,
+                               with_rec ? &c->rec : nullptr, with_rec ? c->rec.n_tasks : 0, 0.5 * c->dt));
+    timed_end(c, e0, true);
+    rec_taken = true;
+    row += grid;
+  }
   for (int medium : {INSEM_MEDIUM_ELASTIC, INSEM_MEDIUM_ACOUSTIC}) {
     const int di = medium == INSEM_MEDIUM_ELASTIC ? c->solid : c->fluid;
     if (di < 0) continue;
     DomainRT& R = c->doms[di];
     for (size_t bi = 0; bi < R.blocks.size(); ++bi) {
       BlockRT& br = R.blocks[bi];
+      if (&br == pa || &br == pb) continue;
       StepParams p;
       if (c->fuse) p = fused_params(sc, corr, pred, clk, c->partial + 2 * row);
       const bool with_rec = c->fuse && !rec_taken;
@@ -1077,7 +1102,7 @@ int create_impl(const insem_hip_setup* s, const insem_hip_batch* batch, insem_hi
       rows += tail_blocks(R.ng) + tail_blocks(R.n_sp) + 2;  // grouped + other regular + special
       for (const BlockRT& br : R.blocks) rows += force_blocks(c->ngll, br.nb);
     }
-    rows += c->rec.n_tasks;  // receiver work-groups of a force launch
+    rows += c->rec.n_tasks + 16;  // receiver work-groups of a force launch, padding of a paired launch
     c->partial_rows = rows;
     int dot_rows = 0;
     for (const DomainRT& R : c->doms) dot_rows = std::max(dot_rows, tail_blocks(R.ncomp * R.ng));
@@ -1087,6 +1112,7 @@ int create_impl(const insem_hip_setup* s, const insem_hip_batch* batch, insem_hi
     HIP_TRY(hipMemset(c->d_clock, 0, sizeof(int32_t)));
     if (const char* g = std::getenv("INSEM_HIP_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
     if (const char* f = std::getenv("INSEM_HIP_FUSE")) c->fuse = std::atoi(f) != 0;
+    if (const char* f = std::getenv("INSEM_HIP_PAIR")) c->pair = std::atoi(f) != 0;
     HIP_TRY(hipDeviceSynchronize());
   } catch (const HipFailure& e) {
     free_all(c);

@@ -614,7 +614,7 @@ __device__ __forceinline__ void write_energy(double ep, double ek, double* red, 
 constexpr int force_waves(int ng, int phys) {
   if (phys == kElasticAxi1) return ng <= 3 ? 8 : ng <= 7 ? 5 : 4;
   if (phys == kElasticAxi0) return ng <= 5 ? 8 : ng <= 8 ? 6 : 5;
-  return 8;
+  return ng <= 7 ? 8 : 7;
 }
 
 template <int NG, int PHYS, bool RECT>
@@ -625,8 +625,8 @@ struct ForceLds {
 };
 
 template <int NG, int PHYS, bool RECT>
-__global__ __launch_bounds__(kWG, force_waves(NG, PHYS)) void k_forces(ForceArgs A, StepParams P, ReceiverDev R, int n_rec,
-                                                double half_dt) {
+__global__ __launch_bounds__(kWG, force_waves(NG, PHYS)) void k_forces(ForceArgs A, StepParams P, ReceiverDev R,
+                                                                       int n_rec, double half_dt) {
   __shared__ __align__(16) unsigned char smem[ForceLds<NG, PHYS, RECT>::bytes];
   __shared__ double red[kWG];
   const int step = P.fuse ? clock_step(P.clk) : 0;
@@ -640,6 +640,44 @@ __global__ __launch_bounds__(kWG, force_waves(NG, PHYS)) void k_forces(ForceArgs
   } else {
     force_batch<NG, PHYS, RECT>(A, P, step, *reinterpret_cast<Tiles<NG, PHYS>*>(smem),
                                 xcd_contiguous(wg - n_rec, gridDim.x - n_rec), energy, ep, ek);
+  }
+  if (energy) write_energy(ep, ek, red, P.partial + 2 * blockIdx.x);
+}
+
+// The elastic block A and the acoustic block B of a time step in one
+// launch (the two are independent: each reads the predicted field of its
+// own domain): the work-groups after the receiver tasks ([0, n_rec), padded
+// to rec_end, a multiple of 8) take A's batches ([rec_end, rec_end + na8),
+// na8 a multiple of 8), then B's, each part in the XCD-contiguous order.
+// B's smaller work-groups then fill the end of A's instead of running as a
+// launch of their own.
+constexpr int pair_waves(int ng, int pa, int pb) {
+  return force_waves(ng, pa) < force_waves(ng, pb) ? force_waves(ng, pa) : force_waves(ng, pb);
+}
+
+template <int NG, int PA, bool RA, int PB, bool RB>
+__global__ __launch_bounds__(kWG, pair_waves(NG, PA, PB)) void k_forces_pair(ForceArgs A, ForceArgs B, int na8, StepParams P, ReceiverDev R, int n_rec, int rec_end,
+              double half_dt) {
+  constexpr size_t la = ForceLds<NG, PA, RA>::bytes, lb = ForceLds<NG, PB, RB>::bytes;
+  __shared__ __align__(16) unsigned char smem[la > lb ? la : lb];
+  __shared__ double red[kWG];
+  const int step = P.fuse ? clock_step(P.clk) : 0;
+  const bool energy = P.fuse && clock_energy(P.clk, step);
+  double ep = 0.0, ek = 0.0;
+  const int wg = blockIdx.x;
+  if (wg < rec_end) {
+    if (wg < n_rec) {
+      double* sw = reinterpret_cast<double*>(smem);
+      float* sx = reinterpret_cast<float*>(smem + kWG * sizeof(double));
+      receiver_task<kWG>(R, wg, P.clk, step - 1, 1, half_dt, sw, sx);
+    }
+  } else if (wg - rec_end < na8) {
+    force_batch<NG, PA, RA>(A, P, step, *reinterpret_cast<Tiles<NG, PA>*>(smem), xcd_contiguous(wg - rec_end, na8),
+                            energy, ep, ek);
+  } else {
+    const int nb8 = gridDim.x - rec_end - na8;
+    force_batch<NG, PB, RB>(B, P, step, *reinterpret_cast<Tiles<NG, PB>*>(smem),
+                            xcd_contiguous(wg - rec_end - na8, nb8), energy, ep, ek);
   }
   if (energy) write_energy(ep, ek, red, P.partial + 2 * blockIdx.x);
 }
@@ -705,6 +743,68 @@ hipError_t launch_gather_mode(int ngll, int phys, bool rect, const ForceArgs& a,
 }
 
 }  // namespace
+
+namespace {
+
+template <int NG, int PA, bool RA, int PB, bool RB>
+hipError_t launch_pair_one(const ForceArgs& a, const ForceArgs& b, const StepParams& p, const RecWork& rw,
+                           hipStream_t s, int* grid_out) {
+  constexpr int EPB = kWG / (NG * NG);
+  auto up8 = [](int n) { return (n + 7) / 8 * 8; };
+  const int ba = (a.nb + EPB - 1) / EPB, bb = (b.nb + EPB - 1) / EPB;
+  const int n_rec = rw.r ? rw.n : 0;
+  const int rec_end = up8(n_rec), na8 = up8(ba);
+  const int grid = rec_end + na8 + bb;
+  if (grid_out) *grid_out = grid;
+  hipLaunchKernelGGL((k_forces_pair<NG, PA, RA, PB, RB>), dim3(grid), dim3(kWG), 0, s, a, b, na8, p,
+                     rw.r ? *rw.r : ReceiverDev(), n_rec, rec_end, rw.half_dt);
+  return hipGetLastError();
+}
+
+template <int NG>
+hipError_t launch_pair_ng(int pa, bool ra, int pb, bool rb, const ForceArgs& a, const ForceArgs& b,
+                          const StepParams& p, const RecWork& w, hipStream_t s, int* g) {
+  if (pa == kElasticCart && pb == kAcousticCart) {
+    if (ra && rb) return launch_pair_one<NG, kElasticCart, true, kAcousticCart, true>(a, b, p, w, s, g);
+    if (ra) return launch_pair_one<NG, kElasticCart, true, kAcousticCart, false>(a, b, p, w, s, g);
+    if (rb) return launch_pair_one<NG, kElasticCart, false, kAcousticCart, true>(a, b, p, w, s, g);
+    return launch_pair_one<NG, kElasticCart, false, kAcousticCart, false>(a, b, p, w, s, g);
+  }
+  if (pa == kElasticAxi0 && pb == kAcousticAxi0)
+    return launch_pair_one<NG, kElasticAxi0, false, kAcousticAxi0, false>(a, b, p, w, s, g);
+  if (pa == kElasticAxi1 && pb == kAcousticAxi1)
+    return launch_pair_one<NG, kElasticAxi1, false, kAcousticAxi1, false>(a, b, p, w, s, g);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+bool forces_pair_supported(int pa, bool ra, int pb, bool rb) {
+  const bool cart = pa == kElasticCart && pb == kAcousticCart;
+  const bool axi = (pa == kElasticAxi0 && pb == kAcousticAxi0) || (pa == kElasticAxi1 && pb == kAcousticAxi1);
+  return cart || (axi && !ra && !rb);
+}
+
+hipError_t launch_forces_pair(int ngll, int pa, bool ra, int pb, bool rb, const ForceArgs& a, const ForceArgs& b,
+                              const StepParams& p, hipStream_t s, int* grid, const ReceiverDev* rec, int n_rec,
+                              double half_dt) {
+  if (!p.fuse || !forces_pair_supported(pa, ra, pb, rb)) return hipErrorInvalidValue;
+  RecWork w;
+  w.r = rec;
+  w.n = n_rec;
+  w.half_dt = half_dt;
+  switch (ngll) {
+    case 2: return launch_pair_ng<2>(pa, ra, pb, rb, a, b, p, w, s, grid);
+    case 3: return launch_pair_ng<3>(pa, ra, pb, rb, a, b, p, w, s, grid);
+    case 4: return launch_pair_ng<4>(pa, ra, pb, rb, a, b, p, w, s, grid);
+    case 5: return launch_pair_ng<5>(pa, ra, pb, rb, a, b, p, w, s, grid);
+    case 6: return launch_pair_ng<6>(pa, ra, pb, rb, a, b, p, w, s, grid);
+    case 7: return launch_pair_ng<7>(pa, ra, pb, rb, a, b, p, w, s, grid);
+    case 8: return launch_pair_ng<8>(pa, ra, pb, rb, a, b, p, w, s, grid);
+    case 9: return launch_pair_ng<9>(pa, ra, pb, rb, a, b, p, w, s, grid);
+  }
+  return hipErrorInvalidValue;
+}
 
 int force_blocks(int ngll, int nb) {
   const int epb = kWG / (ngll * ngll);

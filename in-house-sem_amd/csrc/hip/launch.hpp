@@ -46,6 +46,15 @@ hipError_t launch_forces(int ngll, int phys, bool rect, bool gather, const Force
 // velocity), ahead of the element work-groups.
 // Work-groups of one force launch over nb elements.
 int force_blocks(int ngll, int nb);
+// A time step's force launch over an elastic block a and an acoustic block
+// b together (fused steps only), for the physics pairs of a coupled model:
+// (elastic, acoustic) Cartesian, axisymmetric m = 0 or m = 1.  Receivers as
+// in launch_forces.  The launch has at most 14 work-groups of padding more
+// than the two blocks' force_blocks and n_rec.
+bool forces_pair_supported(int pa, bool ra, int pb, bool rb);
+hipError_t launch_forces_pair(int ngll, int pa, bool ra, int pb, bool rb, const ForceArgs& a, const ForceArgs& b,
+                              const StepParams& p, hipStream_t s, int* grid, const ReceiverDev* rec, int n_rec,
+                              double half_dt);
 
 struct StepCoef {
   float rdt = 0.0f, hdt = 0.0f, hdt2 = 0.0f;  // float(dt), float(dt/2), float(dt^2/2)

@@ -253,16 +253,20 @@ def test_advance_in_chunks_equals_one_call():
             assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("model,switch", [("layered", "INSEM_HIP_FUSE"), ("borehole_m1", "INSEM_HIP_FUSE")])
+@pytest.mark.parametrize("model,switch", [("layered", "INSEM_HIP_FUSE"), ("borehole_m1", "INSEM_HIP_FUSE"),
+                                          ("layered", "INSEM_HIP_PAIR"), ("borehole_m0", "INSEM_HIP_PAIR"),
+                                          ("borehole_m1", "INSEM_HIP_PAIR")])
 def test_step_schedules_are_bitwise_equal(model, switch, monkeypatch):
     # Points owned by one element slot are updated inside the force launch;
     # the same run with every point updated by the tail kernels
-    # (INSEM_HIP_FUSE=0) must give the same bits in traces and fields.
+    # (INSEM_HIP_FUSE=0) must give the same bits in traces and fields.  The
+    # elastic and acoustic blocks share one force launch; launched one after
+    # the other (INSEM_HIP_PAIR=0) they give the same bits too.
     def make():
         if model == "layered":
             return layered_coupled(nx=40, nz_fluid=4, nz_solid=(8, 8), duration=0.03, n_receivers=7,
                                    energy_every=10)
-        return small_borehole(1)
+        return small_borehole(0 if model == "borehole_m0" else 1)
 
     fused = make()
     a = fused.solver
