@@ -464,14 +464,17 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
   HIP_TRY(hipMemset(fl, 0, static_cast<size_t>(d.n_elements) * C * npe * sizeof(float)));
   R.dd.fl = fl;
   R.fl_tmp = c->alloc<float>(static_cast<size_t>(d.n_elements) * C * npe);
-  // Group layout and the rank of each slot of a grouped point.
+  // Group layout, and the code of each slot of a grouped point: its group
+  // and its rank among the point's contributions.
   R.dd.contrib = R.contrib = c->alloc<float>(static_cast<size_t>(R.contrib_len));
   std::vector<uint8_t> slot_rank(static_cast<size_t>(d.n_elements) * npe, 0);
   for (int32_t g = 0; g < d.n_global; ++g) {
     const int64_t gn = R.perm_new[g];
     if (gn < R.dd.grp.begin || gn >= R.dd.grp.end) continue;
+    int q = 0;
+    while (gn >= R.dd.grp.start[q] + R.dd.grp.size[q]) ++q;
     for (int32_t t = d.t_offsets[g]; t < d.t_offsets[g + 1]; ++t)
-      slot_rank[d.t_slots[t]] = static_cast<uint8_t>(t - d.t_offsets[g]);
+      slot_rank[d.t_slots[t]] = slot_code(q, t - d.t_offsets[g]);
   }
 
   // Constraint rows (new numbering, float weights as the reference's gather uses them).
@@ -531,14 +534,18 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
     A.a = R.dd.a;
     A.im = R.dd.im;
     A.mass = R.dd.mass;
-    A.grp = R.dd.grp;
     A.contrib = R.contrib;
+    for (int q = 0; q < R.dd.grp.n; ++q) {
+      A.gdelta[q] = static_cast<int>(R.dd.grp.base[q] - R.dd.grp.start[q]);
+      A.gsize[q] = static_cast<int>(R.dd.grp.size[q]);
+      A.gcount[q] = R.dd.grp.count[q];
+    }
     {
       std::vector<uint8_t> rk(static_cast<size_t>(nb) * npe);
       for (int b = 0; b < nb; ++b)
         for (int k = 0; k < npe; ++k)
           rk[static_cast<size_t>(b) * npe + k] = slot_rank[static_cast<size_t>(B.elements[b]) * npe + k];
-      A.rank = c->upload(rk);
+      A.code = c->upload(rk);
     }
     br.d_elem = c->upload(B.elements, static_cast<size_t>(nb));
     A.elem = br.d_elem;

@@ -88,21 +88,29 @@ struct Groups {
 };
 
 // Group of grouped point g: *off = base + (g - start), *n = size, *cnt =
-// count.  Unrolled selects over the (uniform) table: no indexed loads.
-__device__ __forceinline__ void group_locate(const Groups& G, int64_t g, int64_t* off, int64_t* n, int* cnt) {
-  int64_t b = G.base[0], s = G.start[0], z = G.size[0];
+// count.  Unrolled selects over the (uniform) table: no indexed loads; 32-bit
+// arithmetic (group layouts hold fewer than 2^30 values).
+__device__ __forceinline__ void group_locate(const Groups& G, uint32_t g, uint32_t* off, uint32_t* n, int* cnt) {
+  int32_t d = static_cast<int32_t>(G.base[0] - G.start[0]);
+  uint32_t z = static_cast<uint32_t>(G.size[0]);
   int k = G.count[0];
 #pragma unroll
   for (int q = 1; q < kMaxGroups; ++q) {
-    const bool in = q < G.n && g >= G.start[q];
-    b = in ? G.base[q] : b;
-    s = in ? G.start[q] : s;
-    z = in ? G.size[q] : z;
+    const bool in = q < G.n && g >= static_cast<uint32_t>(G.start[q]);
+    d = in ? static_cast<int32_t>(G.base[q] - G.start[q]) : d;
+    z = in ? static_cast<uint32_t>(G.size[q]) : z;
     k = in ? G.count[q] : k;
   }
-  *off = b + (g - s);
+  *off = g + d;
   *n = z;
   *cnt = k;
+}
+
+// Slot code byte of an element slot whose point is grouped: the flag, the
+// point's group and the slot's rank among the point's contributions.
+constexpr uint8_t kSlotGrouped = 0x80;
+__host__ __device__ constexpr uint8_t slot_code(int group, int rank) {
+  return static_cast<uint8_t>(kSlotGrouped | (group << 3) | rank);
 }
 
 // Time-step coefficients of one model of a batch, computed as a single
@@ -153,8 +161,9 @@ struct ForceArgs {
   const float* mass = nullptr;
   // Slots of grouped points write their force into the group layout
   // instead of f_local (fused steps only).
-  Groups grp;
-  const uint8_t* rank = nullptr;  // [nb][npe]
+  const uint8_t* code = nullptr;  // [nb][npe] slot codes (slot_code), 0: not grouped
+  // Per group (the code's group index): base - start, size, count.
+  int gdelta[kMaxGroups] = {}, gsize[kMaxGroups] = {}, gcount[kMaxGroups] = {};
   float* contrib = nullptr;
   // Batches: model of each block element and the models' coefficients.
   const int32_t* emodel = nullptr;

@@ -441,7 +441,7 @@ struct Slot {
   int b = 0;
   int32_t m = -1;  // global point (< 0: constrained slot of row -1 - m)
   Ops o;
-  int rank = 0;
+  int code = 0;  // slot code byte (grouped flag, group, rank)
   bool owned = false;
   float uu[C];
   float pv[C];
@@ -463,7 +463,7 @@ __device__ __forceinline__ void stage_point(const ForceArgs& A, const StepParams
     const uint32_t slot = b * NPE + k;
     s.m = *at32(A.l2g, slot);
     load_ops<PHYS, RECT>(A, b, slot, s.o);
-    s.rank = *at32(A.rank, slot);
+    s.code = *at32(A.code, slot);
   }
 }
 
@@ -540,13 +540,22 @@ __device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepPara
     }
     return;
   }
-  if (P.fuse && s.m >= A.grp.begin && s.m < A.grp.end) {
-    int64_t off, n;
-    int cnt;
-    group_locate(A.grp, s.m, &off, &n, &cnt);
-    const uint32_t q = static_cast<uint32_t>(off) + s.rank * static_cast<uint32_t>(n);
+  if (P.fuse && (s.code & kSlotGrouped)) {
+    // The slot's place in the group layout: base + (g - start) + rank size,
+    // components size * count apart (selects over the uniform table).
+    const int qi = (s.code >> 3) & 7;
+    int4 G = make_int4(A.gdelta[0], A.gsize[0], A.gcount[0], 0);
 #pragma unroll
-    for (int c = 0; c < C; ++c) *at32(A.contrib, q + c * cnt * static_cast<uint32_t>(n)) = fo[c];
+    for (int q = 1; q < kMaxGroups; ++q) {
+      const bool in = qi == q;
+      G.x = in ? A.gdelta[q] : G.x;
+      G.y = in ? A.gsize[q] : G.y;
+      G.z = in ? A.gcount[q] : G.z;
+    }
+    const uint32_t q = static_cast<uint32_t>(s.m) + G.x + (s.code & 7) * G.y;
+    const uint32_t cs = static_cast<uint32_t>(G.y * G.z);
+#pragma unroll
+    for (int c = 0; c < C; ++c) *at32(A.contrib, q + c * cs) = fo[c];
     return;
   }
   const uint32_t out = (static_cast<uint32_t>(A.out_pos0) + s.b) * (C * NPE) + k;
@@ -614,7 +623,7 @@ __device__ __forceinline__ void write_energy(double ep, double ek, double* red, 
 constexpr int force_waves(int ng, int phys) {
   if (phys == kElasticAxi1) return ng <= 3 ? 8 : ng <= 7 ? 5 : 4;
   if (phys == kElasticAxi0) return ng <= 5 ? 8 : ng <= 8 ? 6 : 5;
-  return ng <= 7 ? 8 : 7;
+  return ng <= 7 ? 8 : ng == 8 ? 7 : 6;
 }
 
 template <int NG, int PHYS, bool RECT>

@@ -179,10 +179,9 @@ __device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc
   if (g >= d.grp.end) return;
   const PointStep ps = point_step(d, sc, pred, step, g);
   if (!ps.active) return;
-  int64_t off, n;
+  uint32_t o, z;
   int cnt;
-  group_locate(d.grp, g, &off, &n, &cnt);
-  const uint32_t o = static_cast<uint32_t>(off), z = static_cast<uint32_t>(n);
+  group_locate(d.grp, static_cast<uint32_t>(g), &o, &z, &cnt);
   float f[C];
   switch (cnt) {
     case 2: group_sum<C, 2>(d.contrib, o, z, f); break;
