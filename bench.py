@@ -41,7 +41,7 @@ sys.path.insert(0, os.path.join(ROOT, "in-house-sem_amd"))
 HBM_PEAK_GBS = 8000.0
 # HBM traffic of the dominant force launch, measured with rocprofv3 --pmc
 # (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of known size).
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_elastic_forces_layered_800x400.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_forces_pair_layered_800x400.txt")
 
 
 def algorithmic_bytes(image, hip, ctx, check):
@@ -100,17 +100,18 @@ def cpu_baseline(sim, dt, max_seconds=20.0):
     threads = oracle_lib().oracle_set_threads(0)
     img = sim.build_setup()
     o = OracleRun(img, dt=dt)
-    t0 = time.perf_counter()
     o.advance(1)  # initialisation + first step
-    first = time.perf_counter() - t0
-    n = max(2, min(20000, int(max_seconds / max(first, 1e-6))))
+    t0 = time.perf_counter()
+    o.advance(10)  # per-step cost, to size the timed sample
+    per_step = (time.perf_counter() - t0) / 10
+    n = max(2, min(20000, int(max_seconds / max(per_step, 1e-6))))
     t0 = time.perf_counter()
     o.advance(n)
     el = time.perf_counter() - t0
     dof = img.n_dofs
     o.close()
     return {"value": dof * n / el, "unit": "DOF-steps/s", "cores": int(threads), "kind": "port",
-            "sample": f"{n} steps of the same 800x400 order-4 coupled model after 1 warm-up step "
+            "sample": f"{n} steps of the same 800x400 order-4 coupled model after 11 warm-up steps "
                       f"({el:.1f} s, OpenMP oracle on {int(threads)} threads, stepping only)"}
 
 
