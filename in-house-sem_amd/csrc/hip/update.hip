@@ -39,11 +39,10 @@ __device__ double block_sum(double v, double* lds) {
 // slot order.  The row is read in chunks of kChunk slots whose addresses,
 // then values, are all loaded before the chunk's adds.
 template <int C, bool W>
-__device__ __forceinline__ void gather_sum(const DomDev& d, uint32_t g, float* f) {
+__device__ __forceinline__ void gather_rows(const DomDev& d, int32_t s0, int32_t s1, float* f) {
   constexpr int kChunk = 4;
 #pragma unroll
   for (int c = 0; c < C; ++c) f[c] = 0.0f;
-  const int32_t s0 = *at32(d.t_off, g), s1 = *at32(d.t_off, g + 1);
   for (int32_t s = s0; s < s1; s += kChunk) {
     uint32_t at[kChunk];
     float w[kChunk];
@@ -72,6 +71,11 @@ __device__ __forceinline__ void gather_sum(const DomDev& d, uint32_t g, float* f
       }
     }
   }
+}
+
+template <int C, bool W>
+__device__ __forceinline__ void gather_sum(const DomDev& d, uint32_t g, float* f) {
+  gather_rows<C, W>(d, *at32(d.t_off, g), *at32(d.t_off, g + 1), f);
 }
 
 // Coefficients of point g at `step`: the launch's, or on a batch its
@@ -223,93 +227,119 @@ template <int C, bool W>
 __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
                                              int pred, int step, bool energy, int vb, double& ep, double& ek) {
   const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
-  const PointStep ps = i < sp.n_sp ? point_step(d, sc, pred, step, d.n_reg + i) : PointStep{};
-  if (i < sp.n_sp && ps.active) {
-    const int64_t g = d.n_reg + i;
-    const double h_damp = d.models && sc.h_damp > 0.0 ? d.models[d.pmodel[g]].h_damp : sc.h_damp;
-    float f[C];
-    gather_sum<C, W>(d, g, f);
-    float u[C], v[C], a[C];
+  if (i >= sp.n_sp) return;
+  const uint32_t g = static_cast<uint32_t>(d.n_reg + i), ng = static_cast<uint32_t>(d.ng);
+  // The point's loads are issued level by level (first everything indexed
+  // by the point, then what those indices select; coupling operands with a
+  // clamped index), so that each wait covers one level of the chains; the
+  // arithmetic below keeps the reference's order.
+  const int32_t t0 = *at32(d.t_off, g), t1 = *at32(d.t_off, g + 1);
+  const int32_t q0 = sp.src_off[i], q1 = sp.src_off[i + 1];
+  const int32_t ci = sp.cpl_idx[i], ai = sp.abs_idx[i];
+  const uint8_t fl = sp.flags[i];
+  float u[C], v[C], a[C];
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      u[c] = d.u[c * d.ng + g];
-      v[c] = d.v[c * d.ng + g];
-      if (energy) ep += static_cast<double>(u[c]) * static_cast<double>(f[c]);
-    }
-    // Sources, in source order (solver.cpp:679-687).
-    for (int32_t q = sp.src_off[i]; q < sp.src_off[i + 1]; ++q) {
-      const float amp = sp.amps[static_cast<int64_t>(sp.src_id[q]) * sp.amp_stride + step];
+  for (int c = 0; c < C; ++c) {
+    u[c] = *at32(d.u, c * ng + g);
+    v[c] = *at32(d.v, c * ng + g);
+  }
+  const float im = *at32(d.im, g);
+  const double mass = *at32(d.mass, g);
+  const PointStep ps = point_step(d, sc, pred, step, g);
+  float cx = 0.0f, cz = 0.0f;
+  uint32_t o = 0;
+  if (sp.cpl_cx) {
+    const int32_t cq = ci >= 0 ? ci : 0;
+    cx = sp.cpl_cx[cq];
+    cz = sp.cpl_cz[cq];
+    o = static_cast<uint32_t>(sp.cpl_other[cq]);
+  }
+  float cf[C * C];
+  if (ai >= 0) {
 #pragma unroll
-      for (int c = 0; c < C; ++c) f[c] += sp.src_coef[static_cast<int64_t>(q) * C + c] * amp;
+    for (int r = 0; r < C * C; ++r) cf[r] = sp.abs_coef[static_cast<int64_t>(ai) * C * C + r];
+  }
+  float ot[3] = {0.0f, 0.0f, 0.0f};
+  if (ci >= 0) {
+    const uint32_t on = static_cast<uint32_t>(sp.other_ng);
+    ot[0] = *at32(sp.other, o);
+    if (sp.cpl_role != 1) {
+      ot[1] = *at32(sp.other, on + o);
+      if (sp.pm) ot[2] = *at32(sp.other, 2 * on + o);
     }
-    // Coupling (solver.cpp:693-727).
-    const int32_t ci = sp.cpl_idx[i];
-    if (ci >= 0) {
-      const float h = 0.70710678118654752440f;
-      const float cx = sp.cpl_cx[ci], cz = sp.cpl_cz[ci];
-      const int64_t o = sp.cpl_other[ci];
-      if (sp.cpl_role == 1) {
-        if constexpr (C >= 2) {
-          const float p = sp.other[o];
-          if constexpr (C == 3) {
-            f[0] += h * cx * p;
-            f[1] += h * cx * p;
-          } else {
-            f[0] += cx * p;
-          }
-          f[C - 1] += cz * p;
+  }
+  if (!ps.active) return;
+  const double h_damp = d.models && sc.h_damp > 0.0 ? d.models[*at32(d.pmodel, g)].h_damp : sc.h_damp;
+  float f[C];
+  gather_rows<C, W>(d, t0, t1, f);
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+    if (energy) ep += static_cast<double>(u[c]) * static_cast<double>(f[c]);
+  // Sources, in source order (solver.cpp:679-687).
+  for (int32_t q = q0; q < q1; ++q) {
+    const float amp = sp.amps[static_cast<int64_t>(sp.src_id[q]) * sp.amp_stride + step];
+#pragma unroll
+    for (int c = 0; c < C; ++c) f[c] += sp.src_coef[static_cast<int64_t>(q) * C + c] * amp;
+  }
+  // Coupling (solver.cpp:693-727).
+  if (ci >= 0) {
+    const float h = 0.70710678118654752440f;
+    if (sp.cpl_role == 1) {
+      if constexpr (C >= 2) {
+        const float p = ot[0];
+        if constexpr (C == 3) {
+          f[0] += h * cx * p;
+          f[1] += h * cx * p;
+        } else {
+          f[0] += cx * p;
         }
-      } else {
-        const float ar = sp.pm ? h * (sp.other[o] + sp.other[sp.other_ng + o]) : sp.other[o];
-        const float az = sp.other[(sp.pm ? 2 : 1) * sp.other_ng + o];
-        f[0] -= cx * ar + cz * az;
+        f[C - 1] += cz * p;
+      }
+    } else {
+      const float ar = sp.pm ? h * (ot[0] + ot[1]) : ot[0];
+      const float az = sp.pm ? ot[2] : ot[1];
+      f[0] -= cx * ar + cz * az;
+    }
+  }
+  // a = M^-1 f, Stacey points by (M + h C) a = f - C v (solver.cpp:729-762).
+#pragma unroll
+  for (int c = 0; c < C; ++c) a[c] = f[c] * im;
+  if (ai >= 0) {
+    double A[C * C], rhs[C];
+#pragma unroll
+    for (int r = 0; r < C; ++r) {
+      rhs[r] = f[r];
+#pragma unroll
+      for (int s = 0; s < C; ++s) {
+        rhs[r] -= static_cast<double>(cf[r * C + s]) * v[s];
+        A[r * C + s] = h_damp * cf[r * C + s] + (r == s ? mass : 0.0);
       }
     }
-    // a = M^-1 f, Stacey points by (M + h C) a = f - C v (solver.cpp:729-762).
-    const float im = d.im[g];
+    solve_small<C>(A, rhs);
 #pragma unroll
-    for (int c = 0; c < C; ++c) a[c] = f[c] * im;
-    const int32_t ai = sp.abs_idx[i];
-    if (ai >= 0) {
-      const float* cf = sp.abs_coef + static_cast<int64_t>(ai) * C * C;
-      const double m = d.mass[g];
-      double A[C * C], rhs[C];
+    for (int c = 0; c < C; ++c) a[c] = static_cast<float>(rhs[c]);
+  }
 #pragma unroll
-      for (int r = 0; r < C; ++r) {
-        rhs[r] = f[r];
+  for (int c = 0; c < C; ++c) {
+    if ((fl & 1) || ((fl & 2) && ((sp.axis_mask >> c) & 1))) a[c] = 0.0f;
+  }
 #pragma unroll
-        for (int s = 0; s < C; ++s) {
-          rhs[r] -= static_cast<double>(cf[r * C + s]) * v[s];
-          A[r * C + s] = h_damp * cf[r * C + s] + (r == s ? m : 0.0);
-        }
-      }
-      solve_small<C>(A, rhs);
-#pragma unroll
-      for (int c = 0; c < C; ++c) a[c] = static_cast<float>(rhs[c]);
+  for (int c = 0; c < C; ++c) {
+    if (corr) v[c] = v[c] + ps.hdt * a[c];
+    if (energy) {
+      const double vd = v[c], ad = a[c];
+      ek += mass * (0.5 * vd * vd - ps.cc * ad * ad);
     }
-    const uint8_t fl = sp.flags[i];
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      if ((fl & 1) || ((fl & 2) && ((sp.axis_mask >> c) & 1))) a[c] = 0.0f;
+    sp.su[c * sp.n_sp + i] = u[c];
+    sp.sv[c * sp.n_sp + i] = v[c];
+    sp.sa[c * sp.n_sp + i] = a[c];
+    const uint32_t oo = c * ng + g;
+    if (ps.pred) {
+      *at32(d.u, oo) = u[c] + (ps.rdt * v[c] + ps.hdt2 * a[c]);
+      v[c] = v[c] + ps.hdt * a[c];
     }
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      if (corr) v[c] = v[c] + ps.hdt * a[c];
-      if (energy) {
-        const double vd = v[c], ad = a[c];
-        ek += static_cast<double>(d.mass[g]) * (0.5 * vd * vd - ps.cc * ad * ad);
-      }
-      sp.su[c * sp.n_sp + i] = u[c];
-      sp.sv[c * sp.n_sp + i] = v[c];
-      sp.sa[c * sp.n_sp + i] = a[c];
-      const int64_t o = c * d.ng + g;
-      if (ps.pred) {
-        d.u[o] = u[c] + (ps.rdt * v[c] + ps.hdt2 * a[c]);
-        v[c] = v[c] + ps.hdt * a[c];
-      }
-      d.v[o] = v[c];
-      d.a[o] = a[c];
-    }
+    *at32(d.v, oo) = v[c];
+    *at32(d.a, oo) = a[c];
   }
 }
 
@@ -317,9 +347,10 @@ __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& 
 // the special points (first, so that their dependent loads start early),
 // the next n_grp_blocks the grouped points, the rest the regular points
 // [reg_first, n_reg) through the CSR.  Energy partials: [block][2].
-// Waves per SIMD of the tail: all eight the hardware holds, for one and two
-// components (the kernel is bound by the latency of its gathers).
-constexpr int tail_waves(int c) { return c <= 2 ? 8 : 5; }
+// Waves per SIMD of the tail (the kernel is bound by the latency of its
+// gathers): all eight the hardware holds for one component; seven for two,
+// where eight would spill scalar registers.
+constexpr int tail_waves(int c) { return c <= 1 ? 8 : c == 2 ? 7 : 5; }
 
 template <int C, bool W>
 __global__ __launch_bounds__(kTB, tail_waves(C)) void k_tail(DomDev d, SpecialDev sp, StepCoef sc, int corr, int pred,
