@@ -140,16 +140,27 @@ __device__ __forceinline__ void load_ops(const ForceArgs& A, uint32_t b, uint32_
   }
 }
 
-// Derivative-matrix rows and columns of point (i, j) (GLJ in xi on axial
-// elements).
+// Derivative-matrix rows (for the gradient) and columns (for the weak
+// divergence) of point (i, j), GLJ in xi on axial elements.  Axisymmetric
+// kernels read the columns after the flux barrier, so that they are not
+// live in registers through the flux computation (which raises their
+// occupancy); the Cartesian ones, at full occupancy either way, read them
+// with the rows.
 template <int NG, int PHYS>
-__device__ __forceinline__ void d_rows(const Tiles<NG, PHYS>& T, bool axial, int i, int j, float* xr, float* yr,
-                                       float* xc, float* yc) {
+__device__ __forceinline__ void d_rows(const Tiles<NG, PHYS>& T, bool axial, int i, int j, float* xr, float* yr) {
   const float* Dx = axial ? T.Dj : T.D;
 #pragma unroll
   for (int l = 0; l < NG; ++l) {
     xr[l] = Dx[i * NG + l];
     yr[l] = T.D[j * NG + l];
+  }
+}
+
+template <int NG, int PHYS>
+__device__ __forceinline__ void d_cols(const Tiles<NG, PHYS>& T, bool axial, int i, int j, float* xc, float* yc) {
+  const float* Dx = axial ? T.Dj : T.D;
+#pragma unroll
+  for (int l = 0; l < NG; ++l) {
     xc[l] = Dx[l * NG + i];
     yc[l] = T.D[l * NG + j];
   }
@@ -419,11 +430,13 @@ __global__ __launch_bounds__(kWG) void k_forces_local(ForceArgs A) {
   }
   __syncthreads();
   float xr[NG], yr[NG], xc[NG], yc[NG];
-  d_rows(T, o.axial, i, j, xr, yr, xc, yc);
+  d_rows(T, o.axial, i, j, xr, yr);
+  if constexpr (!Shape<PHYS>::AXI) d_cols(T, o.axial, i, j, xc, yc);
   float direct0 = 0.0f, direct1 = 0.0f;
   if (act) element_flux<NG, PHYS, RECT>(T, o, le, k, i, j, xr, yr, direct0, direct1);
   __syncthreads();
   if (!act) return;
+  if constexpr (Shape<PHYS>::AXI) d_cols(T, o.axial, i, j, xc, yc);
   float fo[C];
   element_out<NG, PHYS>(T, o.axial, le, k, i, j, xc, yc, direct0, direct1, fo);
   const size_t pos = A.out_ref ? static_cast<size_t>(A.elem[b]) : static_cast<size_t>(A.out_pos0) + b;
@@ -584,11 +597,13 @@ __device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams
   }
   __syncthreads();
   float xr[NG], yr[NG], xc[NG], yc[NG];
-  d_rows(T, s.o.axial, i, j, xr, yr, xc, yc);
+  d_rows(T, s.o.axial, i, j, xr, yr);
+  if constexpr (!Shape<PHYS>::AXI) d_cols(T, s.o.axial, i, j, xc, yc);
   float direct0 = 0.0f, direct1 = 0.0f;
   if (s.act) element_flux<NG, PHYS, RECT>(T, s.o, le, k, i, j, xr, yr, direct0, direct1);
   __syncthreads();
   if (s.act) {
+    if constexpr (Shape<PHYS>::AXI) d_cols(T, s.o.axial, i, j, xc, yc);
     float fo[C];
     element_out<NG, PHYS>(T, s.o.axial, le, k, i, j, xc, yc, direct0, direct1, fo);
     slot_epilogue<C, NPE>(A, P, step, s, k, U, fo, energy, ep, ek);
@@ -621,8 +636,8 @@ __device__ __forceinline__ void write_energy(double ep, double ek, double* red, 
 // kernel waits on two dependent global round trips per work-group, so
 // resident work-groups are what hides them).
 constexpr int force_waves(int ng, int phys) {
-  if (phys == kElasticAxi1) return ng <= 3 ? 8 : ng <= 7 ? 5 : 4;
-  if (phys == kElasticAxi0) return ng <= 5 ? 8 : ng <= 8 ? 6 : 5;
+  if (phys == kElasticAxi1) return ng <= 5 ? 8 : ng == 6 ? 7 : ng == 7 ? 6 : 5;
+  if (phys == kElasticAxi0) return ng <= 6 ? 8 : ng <= 8 ? 7 : 6;
   return ng <= 7 ? 8 : ng == 8 ? 7 : 6;
 }
 
