@@ -54,8 +54,10 @@ def algorithmic_bytes(image, hip, ctx, check):
 
     Force work of a block in a time step: the block's data, u at the points it
     gathers (c per point), and per element slot either the update of its
-    owned point (read v and 1/m, write u and v: 3 c + 1 values) or c forces
-    written for the tail's assembly."""
+    point (owned points and the first slot of batch-local points: read v
+    and 1/m, write u and v: 3 c + 1 values), nothing (the other slots of
+    batch-local points, summed in the work-group) or c forces written for
+    the tail's assembly."""
     npe = image.ngll ** 2
     step = 0
     blocks = []
@@ -68,12 +70,15 @@ def algorithmic_bytes(image, hip, ctx, check):
             metric = {"rect": 3 * ne, "affine": 5 * ne, "curved": 5 * npe * ne, "axisym": 8 * npe * ne}[b["kind"]]
             blk = (metric + coef * npe * ne + npe * ne) * 4
             step += blk
-            owned, read = ctypes.c_int64(), ctypes.c_int64()
+            owned, read, local = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
             check(hip.insem_hip_block_points(ctx, di, bi, ctypes.byref(owned), ctypes.byref(read)))
+            check(hip.insem_hip_block_local_slots(ctx, di, bi, ctypes.byref(local)))
             slots = npe * ne
-            launch = blk + c * read.value * 4 + owned.value * (3 * c + 1) * 4 + (slots - owned.value) * c * 4
+            written = slots - owned.value - local.value
+            launch = blk + c * read.value * 4 + owned.value * (3 * c + 1) * 4 + written * c * 4
             blocks.append({"bytes": launch, "medium": d["medium"], "kind": b["kind"], "n_elements": ne,
-                           "domain": di, "block": bi, "owned_slots": owned.value, "points_read": read.value})
+                           "domain": di, "block": bi, "owned_slots": owned.value, "local_slots": local.value,
+                           "points_read": read.value})
     return step, blocks
 
 
@@ -329,7 +334,7 @@ def main():
                      "kernel": kname, "algorithmic_bytes": kbytes, "avg_launch_ms": k_ms,
                      "launches_per_step": launches_per_step,
                      "blocks": [{k: b[k] for k in ("medium", "kind", "n_elements", "bytes", "owned_slots",
-                                                   "points_read")} for b in kblocks],
+                                                   "local_slots", "points_read")} for b in kblocks],
                      "tail_ms_per_step": um.value / timing_steps, "timing_steps": timing_steps,
                      "step_algorithmic_bytes": step_bytes,
                      "step_frac": step_bytes / (el / args.steps) / (HBM_PEAK_GBS * 1e9)},

@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -70,7 +71,8 @@ struct BlockRT {
   bool rect = false;
   int nb = 0;
   int pos0 = 0;
-  int64_t n_owned = 0;  // slots whose point the fused launch updates
+  int64_t n_owned = 0;  // slots whose point the fused launch updates (owned points, batch-local leaders)
+  int64_t n_local = 0;  // other slots of batch-local points (force kept in LDS)
   int64_t n_read = 0;   // distinct points the launch gathers
   ForceArgs args;
   int32_t* d_elem = nullptr;
@@ -134,6 +136,7 @@ struct insem_hip_ctx {
   int use_graphs = 1;
   int fuse = 1;  // owned points updated inside the force launches
   int pair = 1;  // a step's elastic and acoustic force work in one launch
+  int local = 1;  // points whose slots share one batch are updated in the force launch
   // batch of independent models (0: a single model)
   int n_models = 0;
   ModelCoef* d_models = nullptr;
@@ -334,13 +337,25 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
     if (is_solid) mark(s->coupling.solid_points, s->coupling.n_points);
     if (is_fluid) mark(s->coupling.fluid_points, s->coupling.n_points);
   }
-  // Point classes: 0 owned (one unit-weight slot of its own, not special:
-  // that slot's work-item finishes the point's update), 1 grouped (2 ..
-  // kMaxGroups + 1 unit-weight slots of its own, not special: the slots write
-  // their forces into the group layout), 2 other regular points (assembled
-  // from f_local through the transpose CSR), 3 special.
+  // Launch batches: the element at position p of block b belongs to batch
+  // base_b + p / epb (one work-group's share of a force launch).
+  const int epb = force_batch_elements(s->ngll);
+  std::vector<int32_t> batch_of(static_cast<size_t>(d.n_elements), 0);
+  int32_t n_batches = 0;
+  for (int b = 0; b < d.n_blocks; ++b) {
+    for (int p = 0; p < d.blocks[b].n_elements; ++p) batch_of[d.blocks[b].elements[p]] = n_batches + p / epb;
+    n_batches += (d.blocks[b].n_elements + epb - 1) / epb;
+  }
+  // Point classes: 0 updated inside the force launch: owned (one
+  // unit-weight slot of its own) or batch-local (2 .. kMaxGroups + 1
+  // unit-weight slots of its own, all in one batch: the first slot's
+  // work-item sums them); 1 grouped (2 .. kMaxGroups + 1 unit-weight slots of
+  // its own in several batches: the slots write their forces into the group
+  // layout); 2 other regular points (assembled from f_local through the
+  // transpose CSR); 3 special.  Classes 0 and 1 are never special.
   std::vector<uint8_t> cls(static_cast<size_t>(d.n_global), 2);
   std::vector<uint8_t> cnt(static_cast<size_t>(d.n_global), 0);
+  std::vector<uint8_t> local(static_cast<size_t>(d.n_global), 0);
   for (int32_t g = 0; g < d.n_global; ++g) {
     if (special[g]) {
       cls[g] = 3;
@@ -351,7 +366,12 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
     for (int32_t t = d.t_offsets[g]; own && t < d.t_offsets[g + 1]; ++t)
       own = d.l2g[d.t_slots[t]] == g && (!d.t_weights || static_cast<float>(d.t_weights[t]) == 1.0f);
     if (own) {
-      cls[g] = n == 1 ? 0 : 1;
+      bool one_batch = c->local && n >= 2;
+      const int32_t b0 = batch_of[d.t_slots[d.t_offsets[g]] / npe];
+      for (int32_t t = d.t_offsets[g]; one_batch && t < d.t_offsets[g + 1]; ++t)
+        one_batch = batch_of[d.t_slots[t] / npe] == b0;
+      local[g] = one_batch;
+      cls[g] = n == 1 || one_batch ? 0 : 1;
       cnt[g] = static_cast<uint8_t>(n);
     }
   }
@@ -464,17 +484,26 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
   HIP_TRY(hipMemset(fl, 0, static_cast<size_t>(d.n_elements) * C * npe * sizeof(float)));
   R.dd.fl = fl;
   R.fl_tmp = c->alloc<float>(static_cast<size_t>(d.n_elements) * C * npe);
-  // Group layout, and the code of each slot of a grouped point: its group
-  // and its rank among the point's contributions.
+  // Group layout, and the code of each slot of a grouped point (its group
+  // and its rank among the point's contributions) or of a batch-local point
+  // (leader, count, place in the batch's force buffer).
   R.dd.contrib = R.contrib = c->alloc<float>(static_cast<size_t>(R.contrib_len));
-  std::vector<uint8_t> slot_rank(static_cast<size_t>(d.n_elements) * npe, 0);
+  std::vector<uint16_t> slot_rank(static_cast<size_t>(d.n_elements) * npe, 0);
+  std::vector<int32_t> batch_fill(static_cast<size_t>(n_batches), 0);
   for (int32_t g = 0; g < d.n_global; ++g) {
+    const int32_t t0 = d.t_offsets[g], n = d.t_offsets[g + 1] - t0;
+    if (local[g]) {
+      const int32_t bt = batch_of[d.t_slots[t0] / npe];
+      const int32_t z0 = batch_fill[bt];
+      batch_fill[bt] += n;  // at most epb * npe <= 256 places per batch
+      for (int32_t r = 0; r < n; ++r) slot_rank[d.t_slots[t0 + r]] = local_code(r == 0, n, z0 + r);
+      continue;
+    }
     const int64_t gn = R.perm_new[g];
     if (gn < R.dd.grp.begin || gn >= R.dd.grp.end) continue;
     int q = 0;
     while (gn >= R.dd.grp.start[q] + R.dd.grp.size[q]) ++q;
-    for (int32_t t = d.t_offsets[g]; t < d.t_offsets[g + 1]; ++t)
-      slot_rank[d.t_slots[t]] = slot_code(q, t - d.t_offsets[g]);
+    for (int32_t r = 0; r < n; ++r) slot_rank[d.t_slots[t0 + r]] = slot_code(q, r);
   }
 
   // Constraint rows (new numbering, float weights as the reference's gather uses them).
@@ -509,7 +538,10 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
       for (int k = 0; k < npe; ++k) {
         const int32_t m = d.l2g[static_cast<size_t>(B.elements[b]) * npe + k];
         l2g[static_cast<size_t>(b) * npe + k] = m >= 0 ? R.perm_new[m] : m;
-        br.n_owned += m >= 0 && R.perm_new[m] < R.n_int;
+        const uint16_t code = slot_rank[static_cast<size_t>(B.elements[b]) * npe + k];
+        const bool lead = !(code & kSlotLocal) || (code & kSlotLeader);
+        br.n_owned += m >= 0 && R.perm_new[m] < R.n_int && lead;
+        br.n_local += (code & kSlotLocal) && !(code & kSlotLeader);
         if (m >= 0) {
           br.n_read += !seen[m];
           seen[m] = 1;
@@ -541,10 +573,12 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
       A.gcount[q] = R.dd.grp.count[q];
     }
     {
-      std::vector<uint8_t> rk(static_cast<size_t>(nb) * npe);
+      std::vector<uint16_t> rk(static_cast<size_t>(nb) * npe);
       for (int b = 0; b < nb; ++b)
-        for (int k = 0; k < npe; ++k)
+        for (int k = 0; k < npe; ++k) {
           rk[static_cast<size_t>(b) * npe + k] = slot_rank[static_cast<size_t>(B.elements[b]) * npe + k];
+          A.has_local = A.has_local || (rk[static_cast<size_t>(b) * npe + k] & kSlotLocal);
+        }
       A.code = c->upload(rk);
     }
     br.d_elem = c->upload(B.elements, static_cast<size_t>(nb));
@@ -1076,15 +1110,112 @@ int insem_hip_create_batch(const insem_hip_setup* s, const insem_hip_batch* batc
 
 namespace {
 
-int create_impl(const insem_hip_setup* s, const insem_hip_batch* batch, insem_hip_ctx** out) {
+// Launch order of one block's elements: batches of epb elements (one
+// work-group's share) grown greedily from the first unplaced element in
+// block order, each time adding the unplaced element of the block with the
+// most slot incidences on points the batch already touches (ties: block
+// order).  Batches come out compact (about square patches on a structured
+// mesh instead of strips), so more points have all their slots in one
+// batch.  Results do not depend on the order: every sum runs in the
+// reference's slot order.
+std::vector<int32_t> batch_order(const insem_hip_domain& d, const insem_hip_block& B, int npe, int epb) {
+  const int nb = B.n_elements;
+  std::vector<int32_t> loc(static_cast<size_t>(d.n_elements), -1);
+  for (int i = 0; i < nb; ++i) loc[B.elements[i]] = i;
+  std::vector<int32_t> order, score(static_cast<size_t>(nb), 0), cand;
+  std::vector<uint8_t> placed(static_cast<size_t>(nb), 0);
+  order.reserve(nb);
+  int seed = 0;
+  auto place = [&](int x) {
+    placed[x] = 1;
+    order.push_back(x);
+    const int32_t* row = d.l2g + static_cast<size_t>(B.elements[x]) * npe;
+    for (int k = 0; k < npe; ++k) {
+      if (row[k] < 0) continue;
+      for (int32_t t = d.t_offsets[row[k]]; t < d.t_offsets[row[k] + 1]; ++t) {
+        const int32_t y = loc[d.t_slots[t] / npe];
+        if (y < 0 || placed[y]) continue;
+        if (score[y]++ == 0) cand.push_back(y);
+      }
+    }
+  };
+  while (static_cast<int>(order.size()) < nb) {
+    for (int n = 0; n < epb && static_cast<int>(order.size()) < nb; ++n) {
+      int best = -1;
+      for (int y : cand)
+        if (!placed[y] && (best < 0 || score[y] > score[best] || (score[y] == score[best] && y < best))) best = y;
+      if (best < 0) {
+        while (placed[seed]) ++seed;
+        best = seed;
+      }
+      place(best);
+    }
+    for (int y : cand) score[y] = 0;
+    cand.clear();
+  }
+  return order;
+}
+
+// A copy of a setup whose blocks list their elements (and per-element data)
+// in batch_order.
+struct BatchOrderedSetup {
+  insem_hip_setup setup{};
+  std::vector<insem_hip_domain> domains;
+  std::vector<std::vector<insem_hip_block>> blocks;
+  std::vector<std::unique_ptr<std::vector<int32_t>>> ints;
+  std::vector<std::unique_ptr<std::vector<float>>> floats;
+  std::vector<std::unique_ptr<std::vector<uint8_t>>> bytes;
+
+  template <class T>
+  const T* permuted(const T* src, const std::vector<int32_t>& order, size_t stride,
+                    std::vector<std::unique_ptr<std::vector<T>>>& keep) {
+    if (!src) return nullptr;
+    auto v = std::make_unique<std::vector<T>>(order.size() * stride);
+    for (size_t i = 0; i < order.size(); ++i)
+      std::copy(src + order[i] * stride, src + (order[i] + 1) * stride, v->data() + i * stride);
+    keep.push_back(std::move(v));
+    return keep.back()->data();
+  }
+
+  explicit BatchOrderedSetup(const insem_hip_setup* s) : setup(*s) {
+    const int npe = s->ngll * s->ngll, epb = force_batch_elements(s->ngll);
+    domains.assign(s->domains, s->domains + s->n_domains);
+    blocks.resize(s->n_domains);
+    for (int di = 0; di < s->n_domains; ++di) {
+      const insem_hip_domain& d = s->domains[di];
+      blocks[di].assign(d.blocks, d.blocks + d.n_blocks);
+      for (insem_hip_block& B : blocks[di]) {
+        const std::vector<int32_t> order = batch_order(d, B, npe, epb);
+        const size_t ms = B.kind == INSEM_KIND_RECT ? 3 : B.kind == INSEM_KIND_AFFINE ? 5 : 5 * npe;
+        B.elements = permuted(B.elements, order, 1, ints);
+        B.metric = permuted(B.metric, order, ms, floats);
+        B.coef0 = permuted(B.coef0, order, npe, floats);
+        B.coef1 = permuted(B.coef1, order, npe, floats);
+        B.wr = permuted(B.wr, order, npe, floats);
+        B.hoop = permuted(B.hoop, order, npe, floats);
+        B.r = permuted(B.r, order, npe, floats);
+        B.axial = permuted(B.axial, order, 1, bytes);
+      }
+      domains[di].blocks = blocks[di].data();
+    }
+    setup.domains = domains.data();
+  }
+};
+
+int create_impl(const insem_hip_setup* s0, const insem_hip_batch* batch, insem_hip_ctx** out) {
   if (!out) return set_error(INSEM_HIP_EINVAL, "null output pointer");
   *out = nullptr;
-  int v = validate_setup(s);
-  if (v == INSEM_HIP_OK && batch) v = validate_batch(s, batch);
+  int v = validate_setup(s0);
+  if (v == INSEM_HIP_OK && batch) v = validate_batch(s0, batch);
   if (v != INSEM_HIP_OK) return v;
   if (insem_hip_device_count() <= 0) return set_error(INSEM_HIP_EDEVICE, "no HIP device is visible");
   auto* c = new insem_hip_ctx();
   try {
+    // Blocks are launched in batch order unless INSEM_HIP_BATCH_ORDER=0.
+    const char* bo = std::getenv("INSEM_HIP_BATCH_ORDER");
+    std::unique_ptr<BatchOrderedSetup> ordered;
+    if (!bo || std::atoi(bo) != 0) ordered = std::make_unique<BatchOrderedSetup>(s0);
+    const insem_hip_setup* s = ordered ? &ordered->setup : s0;
     HIP_TRY(hipGetDevice(&c->device));
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->ngll = s->ngll;
@@ -1097,6 +1228,7 @@ int create_impl(const insem_hip_setup* s, const insem_hip_batch* batch, insem_hi
       if (s->domains[di].medium == INSEM_MEDIUM_ELASTIC) c->solid = di;
       if (s->domains[di].medium == INSEM_MEDIUM_ACOUSTIC) c->fluid = di;
     }
+    if (const char* f = std::getenv("INSEM_HIP_LOCAL")) c->local = std::atoi(f) != 0;
     for (int di = 0; di < s->n_domains; ++di) build_domain(c, s, di);
     build_coupling(c, s);
     build_receivers(c, s);
@@ -1552,6 +1684,15 @@ int insem_hip_block_points(insem_hip_ctx* c, int32_t di, int32_t bi, int64_t* ow
     return set_error(INSEM_HIP_EINVAL, "no block %d of domain %d", bi, di);
   *owned = c->fuse ? c->doms[di].blocks[bi].n_owned : 0;
   *read = c->doms[di].blocks[bi].n_read;
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_block_local_slots(insem_hip_ctx* c, int32_t di, int32_t bi, int64_t* local_slots) {
+  if (!c || !local_slots) return set_error(INSEM_HIP_EINVAL, "null argument");
+  if (di < 0 || di >= static_cast<int32_t>(c->doms.size()) || bi < 0 ||
+      bi >= static_cast<int32_t>(c->doms[di].blocks.size()))
+    return set_error(INSEM_HIP_EINVAL, "no block %d of domain %d", bi, di);
+  *local_slots = c->fuse ? c->doms[di].blocks[bi].n_local : 0;
   return INSEM_HIP_OK;
 }
 

@@ -106,11 +106,21 @@ __device__ __forceinline__ void group_locate(const Groups& G, uint32_t g, uint32
   *cnt = k;
 }
 
-// Slot code byte of an element slot whose point is grouped: the flag, the
-// point's group and the slot's rank among the point's contributions.
-constexpr uint8_t kSlotGrouped = 0x80;
-__host__ __device__ constexpr uint8_t slot_code(int group, int rank) {
-  return static_cast<uint8_t>(kSlotGrouped | (group << 3) | rank);
+// Slot codes (16 bits per element slot; 0 for any other slot).
+// Grouped point: the flag, the point's group and the slot's rank among the
+// point's contributions.  Batch-local point (all of its slots in one batch
+// of one force launch): the flag, whether the slot is the point's first in
+// slot order (its leader), the point's slot count - 1 and the slot's place z
+// in the batch's force buffer (the point's slots take consecutive places in
+// slot order).
+constexpr uint16_t kSlotGrouped = 0x8000;
+constexpr uint16_t kSlotLocal = 0x4000;
+constexpr uint16_t kSlotLeader = 0x0800;
+__host__ __device__ constexpr uint16_t slot_code(int group, int rank) {
+  return static_cast<uint16_t>(kSlotGrouped | (group << 3) | rank);
+}
+__host__ __device__ constexpr uint16_t local_code(bool leader, int count, int z) {
+  return static_cast<uint16_t>(kSlotLocal | (leader ? kSlotLeader : 0) | ((count - 1) << 8) | z);
 }
 
 // Time-step coefficients of one model of a batch, computed as a single
@@ -149,10 +159,12 @@ struct ForceArgs {
   const float* Dj = nullptr;
   const float* w2 = nullptr;
   // Time step with the update fused in (StepParams::fuse): the points
-  // [0, n_int) each belong to one element slot only (multiplicity 1,
-  // unconstrained, no boundary or source work), so that slot's force is the
-  // assembled force and the work-item finishes the point's update (M^-1,
-  // corrector, energy partials, next predictor) instead of writing f_local.
+  // [0, n_int) have unit-weight unconstrained slots of their own in one
+  // batch only, and no boundary or source work.  A point of one slot takes
+  // that slot's force as its assembled force; a batch-local point's slots
+  // leave their forces in LDS and its first slot sums them.  That work-item
+  // finishes the point's update (M^-1, corrector, energy partials, next
+  // predictor) instead of writing f_local.
   int64_t n_int = 0;
   float* uw = nullptr;
   float* v = nullptr;
@@ -161,7 +173,8 @@ struct ForceArgs {
   const float* mass = nullptr;
   // Slots of grouped points write their force into the group layout
   // instead of f_local (fused steps only).
-  const uint8_t* code = nullptr;  // [nb][npe] slot codes (slot_code), 0: not grouped
+  const uint16_t* code = nullptr;  // [nb][npe] slot codes (slot_code, local_code)
+  int has_local = 0;               // some slot of the block is batch-local
   // Per group (the code's group index): base - start, size, count.
   int gdelta[kMaxGroups] = {}, gsize[kMaxGroups] = {}, gcount[kMaxGroups] = {};
   float* contrib = nullptr;

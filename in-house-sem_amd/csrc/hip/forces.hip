@@ -105,6 +105,7 @@ struct Tiles {
   float D[NPE], Dj[NPE], W2[NPE];
   float u[EPB * C * NPE];
   float f[EPB * 2 * C * NPE];
+  float fo[kWG * C];  // forces of batch-local slots, [z][c]
   float g[AXI ? EPB * NAX * NG : 1];
   float xr[AXI ? EPB * NG : 1];
   float er[AXI ? EPB * NG : 1];
@@ -454,8 +455,8 @@ struct Slot {
   int b = 0;
   int32_t m = -1;  // global point (< 0: constrained slot of row -1 - m)
   Ops o;
-  int code = 0;  // slot code byte (grouped flag, group, rank)
-  bool owned = false;
+  int code = 0;  // slot code (grouped / batch-local)
+  bool upd = false;  // the slot updates its point (owned point or batch-local leader)
   float uu[C];
   float pv[C];
   float pim = 0.0f;
@@ -491,8 +492,8 @@ __device__ __forceinline__ void stage_gather(const ForceArgs& A, int fuse, Slot<
   const uint32_t mg = m >= 0 ? m : 0;
 #pragma unroll
   for (int c = 0; c < C; ++c) s.uu[c] = *at32(A.u, c * ng + mg);
-  s.owned = fuse && m >= 0 && m < A.n_int;
-  const uint32_t mo = s.owned ? m : 0;
+  s.upd = fuse && m >= 0 && m < A.n_int && (!(s.code & kSlotLocal) || (s.code & kSlotLeader));
+  const uint32_t mo = s.upd ? m : 0;
 #pragma unroll
   for (int c = 0; c < C; ++c) s.pv[c] = *at32(A.v, c * ng + mo);
   s.pim = *at32(A.im, mo);
@@ -507,50 +508,67 @@ __device__ __forceinline__ void stage_gather(const ForceArgs& A, int fuse, Slot<
   }
 }
 
-// Consumes the slot's force fo.
+// Update of the slot's point with its assembled force f: a = f m^-1,
+// corrector, energy terms, next predictor, following the tail's
+// expressions in the same order (update.hip), with the model's own
+// coefficients on a batch (whose last step predicts nothing).
+template <int C, int NPE>
+__device__ __forceinline__ void point_update(const ForceArgs& A, const StepParams& P, int step, const Slot<C>& s,
+                                             int k, const float* U, const float* f, bool energy, double& ep,
+                                             double& ek) {
+  const uint32_t g = s.m;
+  const uint32_t ng = static_cast<uint32_t>(A.ng);
+  float rdt = P.rdt, hdt = P.hdt, hdt2 = P.hdt2;
+  double cc = P.cc;
+  int pred = P.pred;
+  if (A.models) {
+    const ModelCoef& mc = A.models[A.emodel[s.b]];
+    rdt = mc.rdt;
+    hdt = mc.hdt;
+    hdt2 = mc.hdt2;
+    cc = mc.cc;
+    pred = pred && step < mc.n_steps;
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const uint32_t o = c * ng + g;
+    float u = U[c * NPE + k], v = s.pv[c];
+    const float a = f[c] * s.pim;
+    if (energy) ep += static_cast<double>(u) * static_cast<double>(f[c]);
+    if (P.corr) v = v + hdt * a;
+    if (energy) {
+      const double vd = v, ad = a;
+      ek += static_cast<double>(*at32(A.mass, g)) * (0.5 * vd * vd - cc * ad * ad);
+    }
+    if (pred) {
+      u = u + (rdt * v + hdt2 * a);
+      v = v + hdt * a;
+      *at32(A.uw, o) = u;
+    } else {
+      *at32(A.a, o) = a;
+    }
+    *at32(A.v, o) = v;
+  }
+}
+
+// Consumes the slot's force fo.  An owned point's assembled force is its one
+// slot's force (0.0f + f, the assembly's sum of one term); a batch-local
+// slot leaves its force in the batch's buffer fob for the point's leader.
 template <int C, int NPE>
 __device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepParams& P, int step, const Slot<C>& s,
-                                              int k, const float* U, const float* fo, bool energy, double& ep,
-                                              double& ek) {
-  if (s.owned) {
-    // The slot's force is the point's assembled force (0.0f + f, the
-    // assembly's sum of one term); the update follows the tail's
-    // expressions in the same order (update.hip), with the model's own
-    // coefficients on a batch (whose last step predicts nothing).
-    const uint32_t g = s.m;
-    const uint32_t ng = static_cast<uint32_t>(A.ng);
-    float rdt = P.rdt, hdt = P.hdt, hdt2 = P.hdt2;
-    double cc = P.cc;
-    int pred = P.pred;
-    if (A.models) {
-      const ModelCoef& mc = A.models[A.emodel[s.b]];
-      rdt = mc.rdt;
-      hdt = mc.hdt;
-      hdt2 = mc.hdt2;
-      cc = mc.cc;
-      pred = pred && step < mc.n_steps;
-    }
+                                              int k, const float* U, const float* fo, float* fob, bool energy,
+                                              double& ep, double& ek) {
+  if (P.fuse && (s.code & kSlotLocal)) {
+    const int z = s.code & 0xFF;
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const uint32_t o = c * ng + g;
-      float u = U[c * NPE + k], v = s.pv[c];
-      const float f = 0.0f + fo[c];
-      const float a = f * s.pim;
-      if (energy) ep += static_cast<double>(u) * static_cast<double>(f);
-      if (P.corr) v = v + hdt * a;
-      if (energy) {
-        const double vd = v, ad = a;
-        ek += static_cast<double>(*at32(A.mass, g)) * (0.5 * vd * vd - cc * ad * ad);
-      }
-      if (pred) {
-        u = u + (rdt * v + hdt2 * a);
-        v = v + hdt * a;
-        *at32(A.uw, o) = u;
-      } else {
-        *at32(A.a, o) = a;
-      }
-      *at32(A.v, o) = v;
-    }
+    for (int c = 0; c < C; ++c) fob[z * C + c] = fo[c];
+    return;
+  }
+  if (s.upd) {
+    float f[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) f[c] = 0.0f + fo[c];
+    point_update<C, NPE>(A, P, step, s, k, U, f, energy, ep, ek);
     return;
   }
   if (P.fuse && (s.code & kSlotGrouped)) {
@@ -606,7 +624,23 @@ __device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams
     if constexpr (Shape<PHYS>::AXI) d_cols(T, s.o.axial, i, j, xc, yc);
     float fo[C];
     element_out<NG, PHYS>(T, s.o.axial, le, k, i, j, xc, yc, direct0, direct1, fo);
-    slot_epilogue<C, NPE>(A, P, step, s, k, U, fo, energy, ep, ek);
+    slot_epilogue<C, NPE>(A, P, step, s, k, U, fo, T.fo, energy, ep, ek);
+  }
+  if (P.fuse && A.has_local) {
+    // Batch-local points: the leader sums the point's slot forces from 0.0f
+    // in slot order (the assembly's order) and updates the point.
+    __syncthreads();
+    if (s.act && s.upd && (s.code & kSlotLocal)) {
+      const int z = s.code & 0xFF, n = ((s.code >> 8) & 7) + 1;
+      float f[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) f[c] = 0.0f;
+      for (int r = 0; r < n; ++r) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) f[c] += T.fo[(z + r) * C + c];
+      }
+      point_update<C, NPE>(A, P, step, s, k, U, f, energy, ep, ek);
+    }
   }
 }
 
@@ -829,6 +863,8 @@ hipError_t launch_forces_pair(int ngll, int pa, bool ra, int pb, bool rb, const 
   }
   return hipErrorInvalidValue;
 }
+
+int force_batch_elements(int ngll) { return kWG / (ngll * ngll); }
 
 int force_blocks(int ngll, int nb) {
   const int epb = kWG / (ngll * ngll);

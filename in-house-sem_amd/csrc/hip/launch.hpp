@@ -34,7 +34,7 @@ struct ReceiverDev {
 
 // Element forces of one block (gather = read the global field through l2g;
 // otherwise read the element-local field in the reference layout).  With
-// p.fuse (gather only), the owned points [0, a.n_int) are updated in the
+// p.fuse (gather only), the points [0, a.n_int) are updated in the
 // kernel, grouped points receive their forces in the group layout and the
 // energy partials go to p.partial[work-group][2]; *grid (optional) receives
 // the launch's work-group count.
@@ -46,6 +46,8 @@ hipError_t launch_forces(int ngll, int phys, bool rect, bool gather, const Force
 // velocity), ahead of the element work-groups.
 // Work-groups of one force launch over nb elements.
 int force_blocks(int ngll, int nb);
+// Elements per batch (one work-group's share) of a force launch.
+int force_batch_elements(int ngll);
 // A time step's force launch over an elastic block a and an acoustic block
 // b together (fused steps only), for the physics pairs of a coupled model:
 // (elastic, acoustic) Cartesian, axisymmetric m = 0 or m = 1.  Receivers as

@@ -69,6 +69,7 @@ def hip():
         _declare(lib, "insem_hip_kernel_time", c_int, c_void, P(c_double), P(c_int), P(c_double), P(c_int))
         _declare(lib, "insem_hip_block_time", c_int, c_void, c_int, c_int, P(c_double), P(c_int))
         _declare(lib, "insem_hip_block_points", c_int, c_void, c_int, c_int, P(c_i64), P(c_i64))
+        _declare(lib, "insem_hip_block_local_slots", c_int, c_void, c_int, c_int, P(c_i64))
         _declare(lib, "insem_hip_footprint", c_int, c_void, P(c_i64))
         _declare(lib, "insem_hip_create_batch", c_int, c_void, c_void, P(c_void))
         _declare(lib, "insem_hip_set_time_batch", c_int, c_void, P(c_double), P(c_int), P(ctypes.c_float))

@@ -273,13 +273,20 @@ int insem_hip_kernel_time(insem_hip_ctx* ctx, double* force_ms, int32_t* force_l
  * accumulated by the last insem_hip_kernel_time call. */
 int insem_hip_block_time(insem_hip_ctx* ctx, int32_t domain, int32_t block, double* ms, int32_t* launches);
 /* Point counts of one block's force launch in a time step: owned_slots =
- * element slots whose point the launch updates itself (points of one
- * unit-weight slot with no boundary, coupling, source or receiver work; 0
- * when that fusion is off), points_read = distinct global points the launch
+ * element slots that update their point inside the launch (the one slot of
+ * a point with a single unit-weight slot, the first slot of a batch-local
+ * point; points with no boundary, coupling, source or receiver work; 0 when
+ * that fusion is off), points_read = distinct global points the launch
  * gathers.  Used to count the launch's algorithmic bytes; no counterpart in
  * the reference. */
 int insem_hip_block_points(insem_hip_ctx* ctx, int32_t domain, int32_t block, int64_t* owned_slots,
                            int64_t* points_read);
+/* Slots of one block's force launch in a time step whose force stays in the
+ * work-group: the slots of batch-local points (all of whose slots lie in one
+ * work-group's elements) other than the point's first, which sums them and
+ * is counted in owned_slots.  0 when that fusion is off.  Used to count the
+ * launch's algorithmic bytes; no counterpart in the reference. */
+int insem_hip_block_local_slots(insem_hip_ctx* ctx, int32_t domain, int32_t block, int64_t* local_slots);
 
 /* Device bytes held by the context. */
 int insem_hip_footprint(insem_hip_ctx* ctx, int64_t* bytes);

@@ -255,13 +255,18 @@ def test_advance_in_chunks_equals_one_call():
 
 @pytest.mark.parametrize("model,switch", [("layered", "INSEM_HIP_FUSE"), ("borehole_m1", "INSEM_HIP_FUSE"),
                                           ("layered", "INSEM_HIP_PAIR"), ("borehole_m0", "INSEM_HIP_PAIR"),
-                                          ("borehole_m1", "INSEM_HIP_PAIR")])
+                                          ("borehole_m1", "INSEM_HIP_PAIR"), ("layered", "INSEM_HIP_LOCAL"),
+                                          ("borehole_m1", "INSEM_HIP_LOCAL"), ("layered", "INSEM_HIP_BATCH_ORDER"),
+                                          ("borehole_m0", "INSEM_HIP_BATCH_ORDER")])
 def test_step_schedules_are_bitwise_equal(model, switch, monkeypatch):
     # Points owned by one element slot are updated inside the force launch;
     # the same run with every point updated by the tail kernels
     # (INSEM_HIP_FUSE=0) must give the same bits in traces and fields.  The
-    # elastic and acoustic blocks share one force launch; launched one after
-    # the other (INSEM_HIP_PAIR=0) they give the same bits too.
+    # elastic and acoustic blocks share one force launch (INSEM_HIP_PAIR=0:
+    # one after the other), points whose slots lie in one work-group are
+    # summed there (INSEM_HIP_LOCAL=0: in the tail), and blocks launch their
+    # elements in compact batches (INSEM_HIP_BATCH_ORDER=0: block order): every
+    # variant gives the same bits.
     def make():
         if model == "layered":
             return layered_coupled(nx=40, nz_fluid=4, nz_solid=(8, 8), duration=0.03, n_receivers=7,
