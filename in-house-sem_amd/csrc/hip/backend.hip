@@ -495,7 +495,7 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
     if (local[g]) {
       const int32_t bt = batch_of[d.t_slots[t0] / npe];
       const int32_t z0 = batch_fill[bt];
-      batch_fill[bt] += n;  // at most epb * npe <= 256 places per batch
+      batch_fill[bt] += n;  // at most epb * npe <= work-group size <= 512 places per batch
       for (int32_t r = 0; r < n; ++r) slot_rank[d.t_slots[t0 + r]] = local_code(r == 0, n, z0 + r);
       continue;
     }

@@ -115,13 +115,16 @@ __device__ __forceinline__ void group_locate(const Groups& G, uint32_t g, uint32
 // slot order).
 constexpr uint16_t kSlotGrouped = 0x8000;
 constexpr uint16_t kSlotLocal = 0x4000;
-constexpr uint16_t kSlotLeader = 0x0800;
+constexpr uint16_t kSlotLeader = 0x1000;
 __host__ __device__ constexpr uint16_t slot_code(int group, int rank) {
   return static_cast<uint16_t>(kSlotGrouped | (group << 3) | rank);
 }
+// z < 512 (9 bits), count 2 .. 8 (3 bits).
 __host__ __device__ constexpr uint16_t local_code(bool leader, int count, int z) {
-  return static_cast<uint16_t>(kSlotLocal | (leader ? kSlotLeader : 0) | ((count - 1) << 8) | z);
+  return static_cast<uint16_t>(kSlotLocal | (leader ? kSlotLeader : 0) | ((count - 1) << 9) | z);
 }
+__host__ __device__ constexpr int local_place(int code) { return code & 0x1FF; }
+__host__ __device__ constexpr int local_count(int code) { return ((code >> 9) & 7) + 1; }
 
 // Time-step coefficients of one model of a batch, computed as a single
 // model's are (float(dt), float(dt/2), float(dt^2/2), dt/2, dt^2/8).

@@ -559,7 +559,7 @@ __device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepPara
                                               int k, const float* U, const float* fo, float* fob, bool energy,
                                               double& ep, double& ek) {
   if (P.fuse && (s.code & kSlotLocal)) {
-    const int z = s.code & 0xFF;
+    const int z = local_place(s.code);
 #pragma unroll
     for (int c = 0; c < C; ++c) fob[z * C + c] = fo[c];
     return;
@@ -631,7 +631,7 @@ __device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams
     // in slot order (the assembly's order) and updates the point.
     __syncthreads();
     if (s.act && s.upd && (s.code & kSlotLocal)) {
-      const int z = s.code & 0xFF, n = ((s.code >> 8) & 7) + 1;
+      const int z = local_place(s.code), n = local_count(s.code);
       float f[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) f[c] = 0.0f;
