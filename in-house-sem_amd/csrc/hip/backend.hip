@@ -956,15 +956,25 @@ void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, 
     prev.offset -= 1;
     HIP_TRY(launch_receivers(c->rec, prev, 1, 0.5 * c->dt, s0));
   }
-  for (int medium : {INSEM_MEDIUM_ELASTIC, INSEM_MEDIUM_ACOUSTIC}) {
-    const int di = medium == INSEM_MEDIUM_ELASTIC ? c->solid : c->fluid;
-    if (di < 0) continue;
-    DomainRT& R = c->doms[di];
+  if (c->fuse && c->pair && c->solid >= 0 && c->fluid >= 0) {
+    DomainRT& S = c->doms[c->solid];
+    DomainRT& F = c->doms[c->fluid];
     hipEvent_t e0 = nullptr;
     timed_begin(c, &e0);
-    HIP_TRY(launch_tail(R.dd, R.sp, sc, corr, pred, clk, c->fuse != 0, c->partial + 2 * row, &nb, s0));
+    HIP_TRY(launch_tail_pair(S.dd, S.sp, F.dd, F.sp, sc, corr, pred, clk, c->partial + 2 * row, &nb, s0));
     timed_end(c, e0, false);
     row += nb;
+  } else {
+    for (int medium : {INSEM_MEDIUM_ELASTIC, INSEM_MEDIUM_ACOUSTIC}) {
+      const int di = medium == INSEM_MEDIUM_ELASTIC ? c->solid : c->fluid;
+      if (di < 0) continue;
+      DomainRT& R = c->doms[di];
+      hipEvent_t e0 = nullptr;
+      timed_begin(c, &e0);
+      HIP_TRY(launch_tail(R.dd, R.sp, sc, corr, pred, clk, c->fuse != 0, c->partial + 2 * row, &nb, s0));
+      timed_end(c, e0, false);
+      row += nb;
+    }
   }
   if (c->energy_every > 0) HIP_TRY(launch_energy_row(c->partial, row, clk, c->energy_rows, s0));
   if (rec_now) HIP_TRY(launch_receivers(c->rec, clk, pv_mode, 0.5 * c->dt, s0));

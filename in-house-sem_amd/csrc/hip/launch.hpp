@@ -74,6 +74,12 @@ int tail_blocks(int64_t n);
 // number and whether it is an energy step come from the clock.
 hipError_t launch_tail(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corrector, int predictor,
                        const StepClock& clk, bool grouped, double* partial, int* n_blocks, hipStream_t s);
+// Fused steps of a coupled model: the tails of both domains in one launch
+// (each interface point's fluid partner is updated right after it, by the
+// same work-item, with its new acceleration).
+hipError_t launch_tail_pair(const DomDev& solid, const SpecialDev& solid_sp, const DomDev& fluid,
+                            const SpecialDev& fluid_sp, const StepCoef& sc, int corrector, int predictor,
+                            const StepClock& clk, double* partial, int* n_blocks, hipStream_t s);
 // rows[step / energy_every] = sums of [n][2] partials, on energy steps only.
 hipError_t launch_energy_row(const double* partial, int n, const StepClock& clk, double* rows, hipStream_t s);
 // *base += n (end of a chunk of n steps).

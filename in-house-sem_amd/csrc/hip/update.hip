@@ -223,11 +223,23 @@ __device__ __forceinline__ void solve_small(double* A, double* b) {
 
 // Special points: sources, coupling, Stacey, Dirichlet and axis conditions;
 // the pre-predictor values go to the side buffers.
+// Coupling operand of a special point (the other domain's values at the
+// partner point) when the caller has it; else it is read from sp.other.
+struct Operand {
+  bool given = false;
+  float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+};
+// The point's new acceleration (components beyond C are 0); active is
+// false when the point's model is past its last step.
+struct Accel {
+  bool active = false;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+};
+
 template <int C, bool W>
-__device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
-                                             int pred, int step, bool energy, int vb, double& ep, double& ek) {
-  const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
-  if (i >= sp.n_sp) return;
+__device__ __forceinline__ Accel special_point(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
+                                               int pred, int step, bool energy, int64_t i, double& ep, double& ek,
+                                               Operand ot_in) {
   const uint32_t g = static_cast<uint32_t>(d.n_reg + i), ng = static_cast<uint32_t>(d.ng);
   // The point's loads are issued level by level (first everything indexed
   // by the point, then what those indices select; coupling operands with a
@@ -261,14 +273,20 @@ __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& 
   }
   float ot[3] = {0.0f, 0.0f, 0.0f};
   if (ci >= 0) {
-    const uint32_t on = static_cast<uint32_t>(sp.other_ng);
-    ot[0] = *at32(sp.other, o);
-    if (sp.cpl_role != 1) {
-      ot[1] = *at32(sp.other, on + o);
-      if (sp.pm) ot[2] = *at32(sp.other, 2 * on + o);
+    if (ot_in.given) {
+      ot[0] = ot_in.v0;
+      ot[1] = ot_in.v1;
+      ot[2] = ot_in.v2;
+    } else {
+      const uint32_t on = static_cast<uint32_t>(sp.other_ng);
+      ot[0] = *at32(sp.other, o);
+      if (sp.cpl_role != 1) {
+        ot[1] = *at32(sp.other, on + o);
+        if (sp.pm) ot[2] = *at32(sp.other, 2 * on + o);
+      }
     }
   }
-  if (!ps.active) return;
+  if (!ps.active) return Accel{};
   const double h_damp = d.models && sc.h_damp > 0.0 ? d.models[*at32(d.pmodel, g)].h_damp : sc.h_damp;
   float f[C];
   gather_rows<C, W>(d, t0, t1, f);
@@ -341,6 +359,56 @@ __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& 
     *at32(d.v, oo) = v[c];
     *at32(d.a, oo) = a[c];
   }
+  Accel r;
+  r.active = true;
+  r.a0 = a[0];
+  if constexpr (C >= 2) r.a1 = a[1];
+  if constexpr (C >= 3) r.a2 = a[2];
+  return r;
+}
+
+template <int C, bool W>
+__device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
+                                             int pred, int step, bool energy, int vb, double& ep, double& ek) {
+  const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
+  if (i >= sp.n_sp) return;
+  special_point<C, W>(d, sp, sc, corr, pred, step, energy, i, ep, ek, Operand{});
+}
+
+// Coupled model, solid special point i: the solid point, then, when it is on
+// the interface, its fluid partner with the solid's new acceleration as the
+// coupling operand (the reference's order: the fluid's coupling term reads
+// the solid's new a, solver.cpp:669-727).  The fluid partner is updated by
+// this work-item only, so the solid's read of its pressure precedes its
+// update.
+template <int CS, bool WS, int CF, bool WF>
+__device__ __forceinline__ void special_pair_body(const DomDev& ds, const SpecialDev& sps, const DomDev& df,
+                                                  const SpecialDev& spf, const StepCoef& sc, int corr, int pred,
+                                                  int step, bool energy, int vb, double& ep, double& ek) {
+  const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
+  if (i >= sps.n_sp) return;
+  const Accel as = special_point<CS, WS>(ds, sps, sc, corr, pred, step, energy, i, ep, ek, Operand{});
+  if (!as.active) return;
+  const int32_t ci = sps.cpl_idx[i];
+  if (ci < 0) return;
+  const int64_t i_f = static_cast<int64_t>(sps.cpl_other[ci]) - df.n_reg;
+  Operand op;
+  op.given = true;
+  op.v0 = as.a0;
+  op.v1 = as.a1;
+  op.v2 = as.a2;
+  special_point<CF, WF>(df, spf, sc, corr, pred, step, energy, i_f, ep, ek, op);
+}
+
+// Coupled model, fluid special points off the interface (the interface ones
+// go with their solid partner).
+template <int C, bool W>
+__device__ __forceinline__ void special_uncoupled_body(const DomDev& d, const SpecialDev& sp, const StepCoef& sc,
+                                                       int corr, int pred, int step, bool energy, int vb, double& ep,
+                                                       double& ek) {
+  const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
+  if (i >= sp.n_sp || sp.cpl_idx[i] >= 0) return;
+  special_point<C, W>(d, sp, sc, corr, pred, step, energy, i, ep, ek, Operand{});
 }
 
 // The tail of a step on one domain, one launch: blocks [0, n_sp_blocks) take
@@ -367,6 +435,36 @@ __global__ __launch_bounds__(kTB, tail_waves(C)) void k_tail(DomDev d, SpecialDe
     grouped_body<C>(d, sc, corr, pred, step, energy, b - n_sp_blocks, ep, ek);
   else
     regular_body<C, W>(d, reg_first, sc, corr, pred, step, energy, b - n_sp_blocks - n_grp_blocks, ep, ek);
+  write_partials(energy, ep, ek, lds, partial);
+}
+
+// The tails of both domains of a coupled model in one launch: blocks
+// [0, nss) the solid's special points (with their fluid partners on the
+// interface), [nss, nss + nsf) the fluid's other special points, then the
+// grouped and regular points of the solid and of the fluid.
+template <int CS, bool WS, int CF, bool WF>
+__global__ __launch_bounds__(kTB, tail_waves(CS)) void k_tail_pair(DomDev ds, SpecialDev sps, DomDev df,
+                                                                   SpecialDev spf, StepCoef sc, int corr, int pred,
+                                                                   StepClock clk, double* partial, int nss, int nsf,
+                                                                   int ngs, int nrs, int ngf) {
+  __shared__ double lds[kTB];
+  const int step = clock_step(clk);
+  const bool energy = clock_energy(clk, step);
+  double ep = 0.0, ek = 0.0;
+  int b = blockIdx.x;
+  if (b < nss) {
+    special_pair_body<CS, WS, CF, WF>(ds, sps, df, spf, sc, corr, pred, step, energy, b, ep, ek);
+  } else if ((b -= nss) < nsf) {
+    special_uncoupled_body<CF, WF>(df, spf, sc, corr, pred, step, energy, b, ep, ek);
+  } else if ((b -= nsf) < ngs) {
+    grouped_body<CS>(ds, sc, corr, pred, step, energy, b, ep, ek);
+  } else if ((b -= ngs) < nrs) {
+    regular_body<CS, WS>(ds, ds.grp.end, sc, corr, pred, step, energy, b, ep, ek);
+  } else if ((b -= nrs) < ngf) {
+    grouped_body<CF>(df, sc, corr, pred, step, energy, b, ep, ek);
+  } else {
+    regular_body<CF, WF>(df, df.grp.end, sc, corr, pred, step, energy, b - ngf, ep, ek);
+  }
   write_partials(energy, ep, ek, lds, partial);
 }
 
@@ -579,6 +677,47 @@ hipError_t launch_tail(const DomDev& d, const SpecialDev& sp, const StepCoef& sc
     default: return hipErrorInvalidValue;
   }
 #undef INSEM_TAIL
+  return hipGetLastError();
+}
+
+namespace {
+
+template <int CS, bool WS, int CF, bool WF>
+void tail_pair_one(const DomDev& ds, const SpecialDev& sps, const DomDev& df, const SpecialDev& spf,
+                   const StepCoef& sc, int corr, int pred, const StepClock& clk, double* partial, int nss, int nsf,
+                   int ngs, int nrs, int ngf, int nrf, hipStream_t s) {
+  hipLaunchKernelGGL((k_tail_pair<CS, WS, CF, WF>), dim3(nss + nsf + ngs + nrs + ngf + nrf), dim3(kTB), 0, s, ds,
+                     sps, df, spf, sc, corr, pred, clk, partial, nss, nsf, ngs, nrs, ngf);
+}
+
+}  // namespace
+
+hipError_t launch_tail_pair(const DomDev& ds, const SpecialDev& sps, const DomDev& df, const SpecialDev& spf,
+                            const StepCoef& sc, int corr, int pred, const StepClock& clk, double* partial,
+                            int* n_blocks, hipStream_t s) {
+  if (df.ncomp != 1 || (ds.ncomp != 2 && ds.ncomp != 3) || sps.cpl_role != 1) return hipErrorInvalidValue;
+  const int nss = blocks_for(sps.n_sp), nsf = blocks_for(spf.n_sp);
+  const int ngs = blocks_for(ds.grp.end - ds.grp.begin), ngf = blocks_for(df.grp.end - df.grp.begin);
+  const int nrs = ds.grp.end < ds.n_reg ? blocks_for(ds.n_reg - ds.grp.end) : 0;
+  const int nrf = df.grp.end < df.n_reg ? blocks_for(df.n_reg - df.grp.end) : 0;
+  const int nb = nss + nsf + ngs + nrs + ngf + nrf;
+  if (n_blocks) *n_blocks = nb;
+  if (nb == 0) return hipSuccess;
+  const bool ws = ds.t_w != nullptr, wf = df.t_w != nullptr;
+#define INSEM_TP(CS, WS, WF) \
+  tail_pair_one<CS, WS, 1, WF>(ds, sps, df, spf, sc, corr, pred, clk, partial, nss, nsf, ngs, nrs, ngf, nrf, s)
+  if (ds.ncomp == 2) {
+    if (ws)
+      wf ? INSEM_TP(2, true, true) : INSEM_TP(2, true, false);
+    else
+      wf ? INSEM_TP(2, false, true) : INSEM_TP(2, false, false);
+  } else {
+    if (ws)
+      wf ? INSEM_TP(3, true, true) : INSEM_TP(3, true, false);
+    else
+      wf ? INSEM_TP(3, false, true) : INSEM_TP(3, false, false);
+  }
+#undef INSEM_TP
   return hipGetLastError();
 }
 
