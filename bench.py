@@ -10,8 +10,8 @@ coupling, sources, update, receivers), with all fields resident in HBM.
 
 Prints one JSON line with the throughput in GLL DOF-steps/s (timed over the
 production path: step chunks replayed as HIP graphs), the roofline of the
-dominant kernel (element forces of the elastic Curved block, with the update
-of the points it owns fused in) from HIP events around each of its launches
+dominant kernel (element forces of both blocks, with the update of the
+points they own fused in) from HIP events around each of its launches
 in a second, directly launched pass of further steps, the HBM traffic of that
 launch measured with rocprofv3 counters (read from the committed summary
 under profiles/), the CPU oracle timed on the host cores on a bounded
@@ -19,7 +19,8 @@ sample (about 20 s) of the same workload, and (under "batched") the solves/s
 and latencies of cased-hole dipole models run as batches against the same
 models run one after another on the plain path, and (under "served") the
 throughput and p50/p99 latency of a Poisson stream of mixed requests served
-with continuous batching against the same stream served one at a time.
+with continuous batching at a sweep of arrival rates, against the same
+stream served one at a time.
 
 `--calibrate-copy` instead runs a plain device-to-device copy of known size
 (the reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters).
@@ -183,19 +184,22 @@ def batched_leg(n_models, duration, seed=2026):
             "batches": len(orders)}
 
 
-def served_leg(n_requests, rate, seed=7):
+def served_leg(n_requests, rate, sweep, seed=7):
     """A Poisson stream of mixed requests (60 % cased-hole dipole, 30 %
     open-hole monopole at 3-8 ms; 10 % 2D layered coupled at 1-4 M nodes for
-    0.5-1 s) served by the continuous-batching loop, then the same stream
-    served one request at a time on the plain path (one slot, no batching).
-    Latency = arrival to traces (and STC for boreholes) ready."""
+    0.5-1 s) served by the continuous-batching loop at each arrival rate of
+    the sweep (the same requests, inter-arrival times scaled), and at `rate`
+    also one request at a time on the plain path (one slot, no batching).
+    Latency = arrival to traces (and STC for boreholes) ready; the rate at
+    which solves/s stops following the arrival rate is the loop's
+    saturation."""
     import numpy as np
 
     from insem_amd import serve
     from insem_amd.configs import cased_hole, layered_coupled, open_hole
 
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    unit_arrivals = np.cumsum(rng.exponential(1.0, n_requests))
     kinds = rng.choice(["cased", "open", "layered"], p=[0.6, 0.3, 0.1], size=n_requests)
     seeds = rng.integers(0, 2**31, size=n_requests)
 
@@ -211,16 +215,26 @@ def served_leg(n_requests, rate, seed=7):
                                n_receivers=101)
 
     stc = dict(slowness=np.linspace(100e-6, 1500e-6, 141), window=5e-4, times=np.linspace(0.0, 2.5e-3, 51))
-    out = {}
-    for name, slots, batch in (("served", 4, 8), ("one_at_a_time", 1, 1)):
-        stream = [(float(t), build(k, s)) for t, k, s in zip(arrivals, kinds, seeds)]
+
+    def run(r, slots, batch):
+        stream = [(float(t) / r, build(k, sd)) for t, k, sd in zip(unit_arrivals, kinds, seeds)]
         srv = serve.Server(max_slots=slots, max_batch=batch, stc=stc)
         t0 = time.perf_counter()
         reqs = srv.serve(stream)
-        out[name] = serve.summary(reqs, time.perf_counter() - t0)
-    out["sample"] = (f"{n_requests} requests, Poisson arrivals at {rate:g}/s (seed {seed}): "
+        return serve.summary(reqs, time.perf_counter() - t0)
+
+    out = {"sweep": []}
+    for r in sorted(set(sweep) | {rate}):
+        res = dict(rate=r, **run(r, 4, 8))
+        out["sweep"].append(res)
+        if r == rate:
+            out["served"] = res
+    out["one_at_a_time"] = dict(rate=rate, **run(rate, 1, 1))
+    out["saturation_solves_per_s"] = max(x["solves_per_s"] for x in out["sweep"])
+    out["sample"] = (f"{n_requests} requests, Poisson arrivals (seed {seed}) at rates {sorted(set(sweep) | {rate})}/s: "
                      f"{int((kinds == 'cased').sum())} cased-hole, {int((kinds == 'open').sum())} open-hole, "
-                     f"{int((kinds == 'layered').sum())} layered; served: 4 slots, batches of up to 8")
+                     f"{int((kinds == 'layered').sum())} layered; served: 4 slots, batches of up to 8; "
+                     f"one at a time at {rate:g}/s")
     return out
 
 
@@ -247,7 +261,9 @@ def main():
     ap.add_argument("--batch-models", type=int, default=24, help="models of the batched leg (0: skip it)")
     ap.add_argument("--batch-duration", type=float, default=5e-3, help="simulated time of each batched model (s)")
     ap.add_argument("--serve-requests", type=int, default=16, help="requests of the served leg (0: skip it)")
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the served leg (1/s)")
+    ap.add_argument("--serve-rate", type=float, default=2.0,
+                    help="Poisson arrival rate of the served leg's comparison with one-at-a-time service (1/s)")
+    ap.add_argument("--serve-sweep", default="1,2,4", help="arrival rates (1/s) the served loop runs at")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every run of this build uses a single MI355X "
@@ -344,7 +360,8 @@ def main():
     if args.batch_models > 0:
         result["batched"] = batched_leg(args.batch_models, args.batch_duration)
     if args.serve_requests > 0:
-        result["served"] = served_leg(args.serve_requests, args.serve_rate)
+        sweep = [float(x) for x in args.serve_sweep.split(",") if x]
+        result["served"] = served_leg(args.serve_requests, args.serve_rate, sweep)
     result["cpu_baseline"] = None if args.no_cpu else cpu_baseline(sim, dt)
     print(json.dumps(result), flush=True)
 
