@@ -16,8 +16,8 @@ in a second, directly launched pass of further steps, the HBM traffic of that
 launch measured with rocprofv3 counters (read from the committed summary
 under profiles/), the CPU oracle timed on the host cores on a bounded
 sample (about 20 s) of the same workload, and (under "batched") the solves/s
-and latencies of cased-hole dipole models run as batches against the same
-models run one after another on the plain path, and (under "served") the
+and latencies of 128 cased-hole dipole models run as batches against a
+sample of them run one after another on the plain path, and (under "served") the
 throughput and p50/p99 latency of a Poisson stream of mixed requests served
 with continuous batching at a sweep of arrival rates, against the same
 stream served one at a time.
@@ -35,6 +35,7 @@ import os
 import re
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "in-house-sem_amd"))
@@ -121,13 +122,15 @@ def cpu_baseline(sim, dt, max_seconds=20.0):
                       f"({el:.1f} s, OpenMP oracle on {int(threads)} threads, stepping only)"}
 
 
-def batched_leg(n_models, duration, seed=2026):
+def batched_leg(n_models, duration, plain_sample, seed=2026):
     """Cased-hole dipole models (azimuthal order 1) drawn from the batched
     workload's distribution, solved end to end (host setup, power-iteration
-    Δt, all steps, traces) one after another on the plain path, then as
-    batches (one per polynomial order) on the same GPU, each batch's first
-    steps queued before the next batch is set up.  Every solve is queued at
-    t = 0; a solve's latency is the time until its traces are ready."""
+    Δt, all steps, traces): all n_models as batches (one per polynomial
+    order) on the GPU, each batch set up while the batches already set up
+    run; and the first plain_sample of the same draw one after another
+    on the plain path, whose solves/s stands for sequential service of the
+    whole set.  Every solve is queued at t = 0; a solve's latency is the
+    time until its traces are ready."""
     import numpy as np
 
     import insem_amd as insem
@@ -137,9 +140,10 @@ def batched_leg(n_models, duration, seed=2026):
         rng = np.random.default_rng(seed)
         return [cased_hole(rng, duration=duration) for _ in range(n_models)]
 
+    n_plain = min(plain_sample, n_models)
     t0 = time.perf_counter()
     plain_done, steps = [], 0
-    for sim in draw():
+    for sim in draw()[:n_plain]:
         s = sim.solver
         s.run()
         _ = s.traces
@@ -152,35 +156,49 @@ def batched_leg(n_models, duration, seed=2026):
     orders = sorted({sim.discretization.order for sim in sims})
     t0 = time.perf_counter()
     batch_done = [0.0] * n_models
-    groups = []
-    for order in orders:
-        idx = [i for i, sim in enumerate(sims) if sim.discretization.order == order]
-        b = insem.BatchSolver([sims[i] for i in idx])
-        b.advance(512)  # queued on the batch's stream: the next batch's setup overlaps these steps
-        groups.append((idx, b))
-    # the batches (one per order) run side by side, each on its own stream
-    live = list(groups)
-    while live:
-        for idx, b in live:
-            b.advance(512)
-        for idx, b in [g for g in live if g[1].step >= g[1].max_steps]:
-            for m in range(b.n_models):
-                _ = b.traces(m)
-            b.close()
-            for i in idx:
-                batch_done[i] = time.perf_counter() - t0
-        live = [g for g in live if g[1]._handle is not None]
+    batch_steps = 0
+    live = []
+    # one batch per order, built on a worker thread (host setup and the
+    # models' power iterations) while the batches already built step on
+    # their own streams, fed 512 steps at a time in turn (the device queues
+    # hold the host back when they are full); a batch's traces are read once
+    # its last step has finished
+    with ThreadPoolExecutor(max_workers=1) as builder:
+        building = [[i for i, sim in enumerate(sims) if sim.discretization.order == order] for order in orders]
+        building = [(idx, builder.submit(insem.BatchSolver, [sims[i] for i in idx])) for idx in building]
+        while building or live:
+            while building and building[0][1].done():
+                idx, f = building.pop(0)
+                b = f.result()
+                batch_steps += sum(b.n_steps(m) for m in range(b.n_models))
+                live.append((idx, b))
+            fed = False
+            for idx, b in live:
+                if b.step < b.max_steps:
+                    b.advance(512)
+                    fed = True
+            for idx, b in [g for g in live if g[1].step >= g[1].max_steps and g[1].idle()]:
+                for m in range(b.n_models):
+                    _ = b.traces(m)
+                b.close()
+                for i in idx:
+                    batch_done[i] = time.perf_counter() - t0
+            live = [g for g in live if g[1]._handle is not None]
+            if not fed:
+                time.sleep(5e-4)
     batch_s = time.perf_counter() - t0
 
     def pct(v, q):
         return float(np.percentile(np.asarray(v) * 1e3, q))
 
-    return {"metric": "batched solves/s", "value": n_models / batch_s, "unit": "solves/s",
-            "plain_solves_per_s": n_models / plain_s, "vs_plain": plain_s / batch_s,
+    value, plain_rate = n_models / batch_s, n_plain / plain_s
+    return {"metric": "batched solves/s", "value": value, "unit": "solves/s",
+            "plain_solves_per_s": plain_rate, "vs_plain": value / plain_rate,
             "latency_ms": {"batched_p50": pct(batch_done, 50), "batched_p99": pct(batch_done, 99),
                            "plain_sequential_p50": pct(plain_done, 50), "plain_sequential_p99": pct(plain_done, 99)},
             "sample": f"{n_models} cased-hole dipole models (seed {seed}, orders {orders}, {duration * 1e3:g} ms each, "
-                      f"{steps} model-steps), end to end",
+                      f"{batch_steps} model-steps) batched, end to end; plain: the first {n_plain} of them "
+                      f"({steps} model-steps) one after another, end to end (latencies of that sequential queue)",
             "batches": len(orders)}
 
 
@@ -258,7 +276,9 @@ def main():
     ap.add_argument("--nx", type=int, default=800)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--calibrate-copy", action="store_true", help="run only a 1 GiB device copy (counter scale)")
-    ap.add_argument("--batch-models", type=int, default=24, help="models of the batched leg (0: skip it)")
+    ap.add_argument("--batch-models", type=int, default=128, help="models of the batched leg (0: skip it)")
+    ap.add_argument("--batch-plain-sample", type=int, default=24,
+                    help="models of the batched leg's sequential plain comparison (the first of the same draw)")
     ap.add_argument("--batch-duration", type=float, default=5e-3, help="simulated time of each batched model (s)")
     ap.add_argument("--serve-requests", type=int, default=16, help="requests of the served leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=2.0,
@@ -358,7 +378,7 @@ def main():
     dt = solver.dt
     solver.close()  # free the model's device memory before the other legs
     if args.batch_models > 0:
-        result["batched"] = batched_leg(args.batch_models, args.batch_duration)
+        result["batched"] = batched_leg(args.batch_models, args.batch_duration, args.batch_plain_sample)
     if args.serve_requests > 0:
         sweep = [float(x) for x in args.serve_sweep.split(",") if x]
         result["served"] = served_leg(args.serve_requests, args.serve_rate, sweep)

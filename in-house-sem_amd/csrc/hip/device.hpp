@@ -184,6 +184,7 @@ struct ForceArgs {
   // Batches: model of each block element and the models' coefficients.
   const int32_t* emodel = nullptr;
   const ModelCoef* models = nullptr;
+  const int32_t* batch_last = nullptr;  // per batch: the last step of its models (a batch's launch skips it after)
 };
 
 // Per-launch parameters of the element-force kernels.
@@ -222,6 +223,7 @@ struct DomDev {
   // single model, whose coefficients come with the launch).
   const int32_t* pmodel = nullptr;
   const ModelCoef* models = nullptr;
+  const int32_t* chunk_last = nullptr;  // per 256 points: the last step of their models
 };
 
 // Special-point data of one domain ([0, n_sp) = points n_reg .. ng - 1).

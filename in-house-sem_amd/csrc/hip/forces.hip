@@ -600,6 +600,9 @@ __device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams
                                             int batch, bool energy, double& ep, double& ek) {
   constexpr int NPE = NG * NG;
   constexpr int C = Shape<PHYS>::C;
+  // On a batch, a batch whose models are all past their last step does
+  // nothing (the whole work-group leaves).
+  if (P.fuse && A.batch_last && batch * (kWG / (NG * NG)) < A.nb && step > A.batch_last[batch]) return;
   const int t = threadIdx.x;
   const int le = t / NPE;
   const int k = t - le * NPE;

@@ -102,6 +102,16 @@ __device__ __forceinline__ PointStep point_step(const DomDev& d, const StepCoef&
   return p;
 }
 
+// On a batch, whether every model of the tail block's points [g0, g0 + kTB)
+// is past its last step at `step` (the block then does nothing).
+__device__ __forceinline__ bool tail_block_done(const DomDev& d, int64_t g0, int step) {
+  if (!d.chunk_last) return false;
+  const int64_t g1 = (g0 + kTB < d.ng ? g0 + kTB : d.ng) - 1;
+  if (g1 < g0) return true;
+  const int32_t l0 = d.chunk_last[g0 >> 8], l1 = d.chunk_last[g1 >> 8];
+  return step > (l0 > l1 ? l0 : l1);
+}
+
 // a = M^-1 f, optional corrector, energy terms and the next predictor at a
 // regular point g with assembled force f.
 template <int C>
@@ -148,8 +158,8 @@ __device__ __forceinline__ void write_partials(bool energy, double ep, double ek
 template <int C, bool W>
 __device__ __forceinline__ void regular_body(const DomDev& d, int64_t first, const StepCoef& sc, int corr, int pred,
                                              int step, bool energy, int vb, double& ep, double& ek) {
-  const int64_t g = first + static_cast<int64_t>(vb) * kTB + threadIdx.x;
-  if (g >= d.n_reg) return;
+  const int64_t g0 = first + static_cast<int64_t>(vb) * kTB, g = g0 + threadIdx.x;
+  if (g >= d.n_reg || tail_block_done(d, g0, step)) return;
   const PointStep ps = point_step(d, sc, pred, step, g);
   if (!ps.active) return;
   float f[C];
@@ -179,8 +189,8 @@ __device__ __forceinline__ void group_sum(const float* contrib, uint32_t off, ui
 template <int C>
 __device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc, int corr, int pred, int step,
                                              bool energy, int vb, double& ep, double& ek) {
-  const int64_t g = d.grp.begin + static_cast<int64_t>(vb) * kTB + threadIdx.x;
-  if (g >= d.grp.end) return;
+  const int64_t g0 = d.grp.begin + static_cast<int64_t>(vb) * kTB, g = g0 + threadIdx.x;
+  if (g >= d.grp.end || tail_block_done(d, g0, step)) return;
   const PointStep ps = point_step(d, sc, pred, step, g);
   if (!ps.active) return;
   uint32_t o, z;
@@ -371,7 +381,7 @@ template <int C, bool W>
 __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
                                              int pred, int step, bool energy, int vb, double& ep, double& ek) {
   const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
-  if (i >= sp.n_sp) return;
+  if (i >= sp.n_sp || tail_block_done(d, d.n_reg + static_cast<int64_t>(vb) * kTB, step)) return;
   special_point<C, W>(d, sp, sc, corr, pred, step, energy, i, ep, ek, Operand{});
 }
 
@@ -386,7 +396,7 @@ __device__ __forceinline__ void special_pair_body(const DomDev& ds, const Specia
                                                   const SpecialDev& spf, const StepCoef& sc, int corr, int pred,
                                                   int step, bool energy, int vb, double& ep, double& ek) {
   const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
-  if (i >= sps.n_sp) return;
+  if (i >= sps.n_sp || tail_block_done(ds, ds.n_reg + static_cast<int64_t>(vb) * kTB, step)) return;
   const Accel as = special_point<CS, WS>(ds, sps, sc, corr, pred, step, energy, i, ep, ek, Operand{});
   if (!as.active) return;
   const int32_t ci = sps.cpl_idx[i];
@@ -407,7 +417,8 @@ __device__ __forceinline__ void special_uncoupled_body(const DomDev& d, const Sp
                                                        int corr, int pred, int step, bool energy, int vb, double& ep,
                                                        double& ek) {
   const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
-  if (i >= sp.n_sp || sp.cpl_idx[i] >= 0) return;
+  if (i >= sp.n_sp || tail_block_done(d, d.n_reg + static_cast<int64_t>(vb) * kTB, step) || sp.cpl_idx[i] >= 0)
+    return;
   special_point<C, W>(d, sp, sc, corr, pred, step, energy, i, ep, ek, Operand{});
 }
 

@@ -46,6 +46,8 @@ def hip():
         lib = ctypes.CDLL(HIP_PATH)
         _declare(lib, "insem_hip_last_error", c_char)
         _declare(lib, "insem_hip_device_count", c_int)
+        _declare(lib, "insem_hip_get_device", c_int, P(c_int))
+        _declare(lib, "insem_hip_set_device", c_int, c_int)
         _declare(lib, "insem_hip_create", c_int, c_void, P(c_void))
         _declare(lib, "insem_hip_destroy", c_int, c_void)
         _declare(lib, "insem_hip_set_time", c_int, c_void, c_double, c_int, P(ctypes.c_float))

@@ -516,6 +516,12 @@ class BatchSolver:
     def run(self) -> None:
         self.advance(self.max_steps - self.step)
 
+    def idle(self) -> bool:
+        """True when every queued step has finished (does not block)."""
+        flag = ctypes.c_int32()
+        check(_lib.hip().insem_hip_query(ctypes.c_void_p(self.context), ctypes.byref(flag)))
+        return bool(flag.value)
+
     def traces(self, m: int) -> Dict[Tuple[str, str], np.ndarray]:
         """Model m's traces, keyed and shaped as Solver.traces of that model."""
         ns = int(self._info(m)[1][1])

@@ -88,9 +88,7 @@ class _Slot:
     footprint: int
 
     def idle(self) -> bool:
-        flag = ctypes.c_int32()
-        _lib.check_hip(_lib.hip().insem_hip_query(ctypes.c_void_p(self.batch.context), ctypes.byref(flag)))
-        return bool(flag.value)
+        return self.batch.idle()
 
 
 class Server:

@@ -180,6 +180,12 @@ const char* insem_hip_last_error(void);
 
 /* Number of visible devices (0 without a GPU; never fails). */
 int insem_hip_device_count(void);
+/* The calling thread's current device, and setting it (contexts are created
+ * on the current device).  No counterpart in the reference, which has no
+ * device; a host that sets models up on several threads makes every thread
+ * use the caller's device with these. */
+int insem_hip_get_device(int32_t* device);
+int insem_hip_set_device(int32_t device);
 
 /*
  * Uploads a setup and allocates the fields (zero).  Replaces the CPU state
