@@ -438,7 +438,6 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
   Groups& G = R.dd.grp;
   G = Groups();
   G.begin = R.n_int;
-  int64_t base = 0;
   for (int n = 2; n <= kMaxGroups + 1; ++n) {
     const int64_t s0 = static_cast<int64_t>(R.perm_old.size());
     for (int32_t g : touch)
@@ -448,12 +447,16 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
     G.count[G.n] = n;
     G.start[G.n] = s0;
     G.size[G.n] = sz;
-    G.base[G.n] = base;
-    base += static_cast<int64_t>(C) * n * sz;
+    G.rows = n;
     ++G.n;
   }
   G.end = static_cast<int64_t>(R.perm_old.size());
-  R.contrib_len = base;
+  R.contrib_len = static_cast<int64_t>(C) * G.rows * (G.end - G.begin);
+  if (R.contrib_len >= kMaxIndex32) {
+    set_error(INSEM_HIP_EINVAL, "domain %d: group rows of %lld values exceed the 32-bit index range", di,
+              static_cast<long long>(R.contrib_len));
+    throw HipFailure{INSEM_HIP_EINVAL};
+  }
   for (int32_t g : touch)
     if (cls[g] == 2) take(g);
   for (int32_t g : touch)
@@ -531,9 +534,7 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
     }
     const int64_t gn = R.perm_new[g];
     if (gn < R.dd.grp.begin || gn >= R.dd.grp.end) continue;
-    int q = 0;
-    while (gn >= R.dd.grp.start[q] + R.dd.grp.size[q]) ++q;
-    for (int32_t r = 0; r < n; ++r) slot_rank[d.t_slots[t0 + r]] = slot_code(q, r);
+    for (int32_t r = 0; r < n; ++r) slot_rank[d.t_slots[t0 + r]] = slot_code(r);
   }
 
   // Constraint rows (new numbering, float weights as the reference's gather uses them).
@@ -597,11 +598,9 @@ void build_domain(insem_hip_ctx* c, const insem_hip_setup* s, int di) {
     A.im = R.dd.im;
     A.mass = R.dd.mass;
     A.contrib = R.contrib;
-    for (int q = 0; q < R.dd.grp.n; ++q) {
-      A.gdelta[q] = static_cast<int>(R.dd.grp.base[q] - R.dd.grp.start[q]);
-      A.gsize[q] = static_cast<int>(R.dd.grp.size[q]);
-      A.gcount[q] = R.dd.grp.count[q];
-    }
+    A.gbegin = static_cast<int>(R.dd.grp.begin);
+    A.gstride = static_cast<int>(R.dd.grp.end - R.dd.grp.begin);
+    A.grows = R.dd.grp.rows;
     {
       std::vector<uint16_t> rk(static_cast<size_t>(nb) * npe);
       for (int b = 0; b < nb; ++b)

@@ -72,43 +72,38 @@ __device__ __forceinline__ bool clock_energy(const StepClock& k, int step) {
 
 // Grouped points of a domain: regular points whose contributions all come
 // from element slots of their own (no constraint weights), numbered in
-// groups of equal contribution count.  Group q holds the points
-// [start[q], start[q] + size[q]), each with count[q] contributions, stored
-// at contrib[base[q] + (c * count[q] + r) * size[q] + (point - start[q])]
-// for component c and rank r (the slot's place in the point's ascending
-// slot order), so that the tail reads them in sequence.
+// groups of equal contribution count: group q holds the points
+// [start[q], start[q] + size[q]), each with count[q] contributions.  The
+// contributions lie in rows of stride = end - begin values: component c,
+// rank r (the slot's place in the point's ascending slot order) of grouped
+// point g at contrib[(c * rows + r) * stride + (g - begin)], rows = the
+// largest count.  A slot so finds its place from its rank alone, and the
+// tail reads every row in sequence.
 constexpr int kMaxGroups = 7;  // contribution counts 2 .. 8
 struct Groups {
   int n = 0;
   int count[kMaxGroups] = {};
   int64_t start[kMaxGroups] = {};
   int64_t size[kMaxGroups] = {};
-  int64_t base[kMaxGroups] = {};
+  int rows = 0;
   int64_t begin = 0, end = 0;  // [begin, end): all grouped points
 };
 
-// Group of grouped point g: *off = base + (g - start), *n = size, *cnt =
-// count.  Unrolled selects over the (uniform) table: no indexed loads; 32-bit
-// arithmetic (group layouts hold fewer than 2^30 values).
-__device__ __forceinline__ void group_locate(const Groups& G, uint32_t g, uint32_t* off, uint32_t* n, int* cnt) {
-  int32_t d = static_cast<int32_t>(G.base[0] - G.start[0]);
-  uint32_t z = static_cast<uint32_t>(G.size[0]);
+// Contribution count of grouped point g: unrolled selects over the
+// (uniform) table, no indexed loads.
+__device__ __forceinline__ int group_count(const Groups& G, uint32_t g) {
   int k = G.count[0];
 #pragma unroll
   for (int q = 1; q < kMaxGroups; ++q) {
     const bool in = q < G.n && g >= static_cast<uint32_t>(G.start[q]);
-    d = in ? static_cast<int32_t>(G.base[q] - G.start[q]) : d;
-    z = in ? static_cast<uint32_t>(G.size[q]) : z;
     k = in ? G.count[q] : k;
   }
-  *off = g + d;
-  *n = z;
-  *cnt = k;
+  return k;
 }
 
 // Slot codes (16 bits per element slot; 0 for any other slot).
-// Grouped point: the flag, the point's group and the slot's rank among the
-// point's contributions.  Batch-local point (all of its slots in one batch
+// Grouped point: the flag and the slot's rank among the point's
+// contributions.  Batch-local point (all of its slots in one batch
 // of one force launch): the flag, whether the slot is the point's first in
 // slot order (its leader), the point's slot count - 1 and the slot's place z
 // in the batch's force buffer (the point's slots take consecutive places in
@@ -116,9 +111,7 @@ __device__ __forceinline__ void group_locate(const Groups& G, uint32_t g, uint32
 constexpr uint16_t kSlotGrouped = 0x8000;
 constexpr uint16_t kSlotLocal = 0x4000;
 constexpr uint16_t kSlotLeader = 0x1000;
-__host__ __device__ constexpr uint16_t slot_code(int group, int rank) {
-  return static_cast<uint16_t>(kSlotGrouped | (group << 3) | rank);
-}
+__host__ __device__ constexpr uint16_t slot_code(int rank) { return static_cast<uint16_t>(kSlotGrouped | rank); }
 // z < 512 (9 bits), count 2 .. 8 (3 bits).
 __host__ __device__ constexpr uint16_t local_code(bool leader, int count, int z) {
   return static_cast<uint16_t>(kSlotLocal | (leader ? kSlotLeader : 0) | ((count - 1) << 9) | z);
@@ -178,8 +171,8 @@ struct ForceArgs {
   // instead of f_local (fused steps only).
   const uint16_t* code = nullptr;  // [nb][npe] slot codes (slot_code, local_code)
   int has_local = 0;               // some slot of the block is batch-local
-  // Per group (the code's group index): base - start, size, count.
-  int gdelta[kMaxGroups] = {}, gsize[kMaxGroups] = {}, gcount[kMaxGroups] = {};
+  // Group rows: first grouped point, row stride, rows per component.
+  int gbegin = 0, gstride = 0, grows = 0;
   float* contrib = nullptr;
   // Batches: model of each block element and the models' coefficients.
   const int32_t* emodel = nullptr;

@@ -529,17 +529,16 @@ __device__ __forceinline__ void point_update(const ForceArgs& A, const StepParam
     cc = mc.cc;
     pred = pred && step < mc.n_steps;
   }
+  float u0[C], vc[C], ac[C];
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const uint32_t o = c * ng + g;
     float u = U[c * NPE + k], v = s.pv[c];
     const float a = f[c] * s.pim;
-    if (energy) ep += static_cast<double>(u) * static_cast<double>(f[c]);
     if (P.corr) v = v + hdt * a;
-    if (energy) {
-      const double vd = v, ad = a;
-      ek += static_cast<double>(*at32(A.mass, g)) * (0.5 * vd * vd - cc * ad * ad);
-    }
+    u0[c] = u;
+    vc[c] = v;
+    ac[c] = a;
     if (pred) {
       u = u + (rdt * v + hdt2 * a);
       v = v + hdt * a;
@@ -549,42 +548,38 @@ __device__ __forceinline__ void point_update(const ForceArgs& A, const StepParam
     }
     *at32(A.v, o) = v;
   }
+  // Energy terms (u.f with the step's u, kinetic with the corrected v), each
+  // sum in component order; all under one uniform branch.
+  if (energy) {
+    const double m = *at32(A.mass, g);
+#pragma unroll
+    for (int c = 0; c < C; ++c) ep += static_cast<double>(u0[c]) * static_cast<double>(f[c]);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const double vd = vc[c], ad = ac[c];
+      ek += m * (0.5 * vd * vd - cc * ad * ad);
+    }
+  }
 }
 
-// Consumes the slot's force fo.  An owned point's assembled force is its one
-// slot's force (0.0f + f, the assembly's sum of one term); a batch-local
-// slot leaves its force in the batch's buffer fob for the point's leader.
+// Stores the slot's force fo where its point's assembly reads it: a
+// batch-local slot in the batch's buffer fob (for the point's leader), a
+// grouped slot in the group layout, any other slot in f_local.  An owned
+// point keeps its force for the update (point_update).
 template <int C, int NPE>
-__device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepParams& P, int step, const Slot<C>& s,
-                                              int k, const float* U, const float* fo, float* fob, bool energy,
-                                              double& ep, double& ek) {
+__device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepParams& P, const Slot<C>& s, int k,
+                                              const float* fo, float* fob) {
   if (P.fuse && (s.code & kSlotLocal)) {
     const int z = local_place(s.code);
 #pragma unroll
     for (int c = 0; c < C; ++c) fob[z * C + c] = fo[c];
     return;
   }
-  if (s.upd) {
-    float f[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) f[c] = 0.0f + fo[c];
-    point_update<C, NPE>(A, P, step, s, k, U, f, energy, ep, ek);
-    return;
-  }
+  if (s.upd) return;
   if (P.fuse && (s.code & kSlotGrouped)) {
-    // The slot's place in the group layout: base + (g - start) + rank size,
-    // components size * count apart (selects over the uniform table).
-    const int qi = (s.code >> 3) & 7;
-    int4 G = make_int4(A.gdelta[0], A.gsize[0], A.gcount[0], 0);
-#pragma unroll
-    for (int q = 1; q < kMaxGroups; ++q) {
-      const bool in = qi == q;
-      G.x = in ? A.gdelta[q] : G.x;
-      G.y = in ? A.gsize[q] : G.y;
-      G.z = in ? A.gcount[q] : G.z;
-    }
-    const uint32_t q = static_cast<uint32_t>(s.m) + G.x + (s.code & 7) * G.y;
-    const uint32_t cs = static_cast<uint32_t>(G.y * G.z);
+    // The slot's place in the group rows: its rank's row, its point's column.
+    const uint32_t q = static_cast<uint32_t>(s.m - A.gbegin) + static_cast<uint32_t>(s.code & 7) * A.gstride;
+    const uint32_t cs = static_cast<uint32_t>(A.grows) * A.gstride;
 #pragma unroll
     for (int c = 0; c < C; ++c) *at32(A.contrib, q + c * cs) = fo[c];
     return;
@@ -623,24 +618,31 @@ __device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams
   float direct0 = 0.0f, direct1 = 0.0f;
   if (s.act) element_flux<NG, PHYS, RECT>(T, s.o, le, k, i, j, xr, yr, direct0, direct1);
   __syncthreads();
+  float fo[C];
   if (s.act) {
     if constexpr (Shape<PHYS>::AXI) d_cols(T, s.o.axial, i, j, xc, yc);
-    float fo[C];
     element_out<NG, PHYS>(T, s.o.axial, le, k, i, j, xc, yc, direct0, direct1, fo);
-    slot_epilogue<C, NPE>(A, P, step, s, k, U, fo, T.fo, energy, ep, ek);
+    slot_epilogue<C, NPE>(A, P, s, k, fo, T.fo);
   }
-  if (P.fuse && A.has_local) {
-    // Batch-local points: the leader sums the point's slot forces from 0.0f
-    // in slot order (the assembly's order) and updates the point.
-    __syncthreads();
-    if (s.act && s.upd && (s.code & kSlotLocal)) {
-      const int z = local_place(s.code), n = local_count(s.code);
+  if (P.fuse) {
+    // The assembled force of an updated point: its one slot's force (0.0f +
+    // f, the assembly's sum of one term), or for a batch-local point the sum
+    // from 0.0f of its slots' forces in slot order (the assembly's order),
+    // taken by the point's leader once the batch has stored them.
+    if (A.has_local) __syncthreads();
+    if (s.act && s.upd) {
       float f[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) f[c] = 0.0f;
-      for (int r = 0; r < n; ++r) {
+      if (s.code & kSlotLocal) {
+        const int z = local_place(s.code), n = local_count(s.code);
+        for (int r = 0; r < n; ++r) {
 #pragma unroll
-        for (int c = 0; c < C; ++c) f[c] += T.fo[(z + r) * C + c];
+          for (int c = 0; c < C; ++c) f[c] += T.fo[(z + r) * C + c];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c) f[c] += fo[c];
       }
       point_update<C, NPE>(A, P, step, s, k, U, f, energy, ep, ek);
     }

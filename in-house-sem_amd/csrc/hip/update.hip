@@ -168,12 +168,15 @@ __device__ __forceinline__ void regular_body(const DomDev& d, int64_t first, con
 }
 
 // Sums of the K contributions per component of a grouped point, from zero
-// in rank order; all C * K loads are issued before the first add.
+// in rank order: rank r of component c in row c * rows + r (stride n); all
+// C * K loads are issued before the first add.
 template <int C, int K>
-__device__ __forceinline__ void group_sum(const float* contrib, uint32_t off, uint32_t n, float* f) {
+__device__ __forceinline__ void group_sum(const float* contrib, uint32_t off, uint32_t n, uint32_t rows, float* f) {
   float x[C * K];
 #pragma unroll
-  for (int q = 0; q < C * K; ++q) x[q] = *at32(contrib, off + q * n);
+  for (int c = 0; c < C; ++c)
+#pragma unroll
+    for (int r = 0; r < K; ++r) x[c * K + r] = *at32(contrib, off + (c * rows + r) * n);
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     float s = 0.0f;
@@ -183,9 +186,8 @@ __device__ __forceinline__ void group_sum(const float* contrib, uint32_t off, ui
   }
 }
 
-// Grouped points: the contributions of a point lie in rank order at stride
-// size[q] in the group layout; the sum starts from zero in slot order as the
-// assembly's does.
+// Grouped points: the contributions of a point lie in rank order in the
+// group rows; the sum starts from zero in slot order as the assembly's does.
 template <int C>
 __device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc, int corr, int pred, int step,
                                              bool energy, int vb, double& ep, double& ek) {
@@ -193,18 +195,17 @@ __device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc
   if (g >= d.grp.end || tail_block_done(d, g0, step)) return;
   const PointStep ps = point_step(d, sc, pred, step, g);
   if (!ps.active) return;
-  uint32_t o, z;
-  int cnt;
-  group_locate(d.grp, static_cast<uint32_t>(g), &o, &z, &cnt);
+  const uint32_t o = static_cast<uint32_t>(g - d.grp.begin);
+  const uint32_t z = static_cast<uint32_t>(d.grp.end - d.grp.begin), rows = static_cast<uint32_t>(d.grp.rows);
   float f[C];
-  switch (cnt) {
-    case 2: group_sum<C, 2>(d.contrib, o, z, f); break;
-    case 3: group_sum<C, 3>(d.contrib, o, z, f); break;
-    case 4: group_sum<C, 4>(d.contrib, o, z, f); break;
-    case 5: group_sum<C, 5>(d.contrib, o, z, f); break;
-    case 6: group_sum<C, 6>(d.contrib, o, z, f); break;
-    case 7: group_sum<C, 7>(d.contrib, o, z, f); break;
-    default: group_sum<C, 8>(d.contrib, o, z, f); break;
+  switch (group_count(d.grp, static_cast<uint32_t>(g))) {
+    case 2: group_sum<C, 2>(d.contrib, o, z, rows, f); break;
+    case 3: group_sum<C, 3>(d.contrib, o, z, rows, f); break;
+    case 4: group_sum<C, 4>(d.contrib, o, z, rows, f); break;
+    case 5: group_sum<C, 5>(d.contrib, o, z, rows, f); break;
+    case 6: group_sum<C, 6>(d.contrib, o, z, rows, f); break;
+    case 7: group_sum<C, 7>(d.contrib, o, z, rows, f); break;
+    default: group_sum<C, 8>(d.contrib, o, z, rows, f); break;
   }
   update_point<C>(d, ps, corr, energy, g, f, ep, ek);
 }
