@@ -1399,6 +1399,15 @@ int insem_hip_set_time_batch(insem_hip_ctx* c, const double* dt, const int32_t* 
         for (int e = 0; e < br.nb; ++e) last[e / epb] = std::max(last[e / epb], n_steps[br.h_emodel[e]]);
         if (!br.args.batch_last) br.args.batch_last = c->upload(last);
         else c->h2d(const_cast<int32_t*>(br.args.batch_last), last.data(), last.size() * sizeof(int32_t));
+        if (!br.args.batch_model) {
+          // the one model of a force batch (-1: elements of several models)
+          std::vector<int32_t> model(last.size(), -2);
+          for (int e = 0; e < br.nb; ++e) {
+            int32_t& x = model[e / epb];
+            x = x == -2 || x == br.h_emodel[e] ? br.h_emodel[e] : -1;
+          }
+          br.args.batch_model = c->upload(model);
+        }
       }
       std::vector<int32_t> last(static_cast<size_t>((R.ng + 255) / 256), -1);
       for (int64_t g = 0; g < R.ng; ++g) last[g >> 8] = std::max(last[g >> 8], n_steps[R.h_pmodel[g]]);

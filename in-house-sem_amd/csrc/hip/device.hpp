@@ -50,9 +50,10 @@ __device__ __forceinline__ T* at32(T* p, uint32_t i) {
   return reinterpret_cast<T*>(reinterpret_cast<B*>(p) + i * static_cast<uint32_t>(sizeof(T)));
 }
 
-// Value of a per-point quantity at slot k of element b; slot = b * npe + k.
-__device__ __forceinline__ float ld(const PF& f, uint32_t b, uint32_t slot) {
-  return *at32(f.p, f.sk ? slot : b);
+// Value of a per-point quantity at slot k of element b, from the byte
+// offsets b4 = 4 b and slot4 = 4 (b * npe + k): one select per quantity.
+__device__ __forceinline__ float ld(const PF& f, uint32_t b4, uint32_t slot4) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(f.p) + (f.sk ? slot4 : b4));
 }
 
 // Step number of a launch: *base (the step before the chunk, on the device)
@@ -178,6 +179,7 @@ struct ForceArgs {
   const int32_t* emodel = nullptr;
   const ModelCoef* models = nullptr;
   const int32_t* batch_last = nullptr;  // per batch: the last step of its models (a batch's launch skips it after)
+  const int32_t* batch_model = nullptr;  // per batch: its one model, or -1 (elements of several models)
 };
 
 // Per-launch parameters of the element-force kernels.

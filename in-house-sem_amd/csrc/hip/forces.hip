@@ -114,9 +114,10 @@ struct Tiles {
 // Per-point operands of an element slot: metric (xi_x, xi_z, eta_x, eta_z,
 // jac; xi_r ... on axisymmetric blocks), coefficients and the axisymmetric
 // factors.
+// (Set by load_ops for active slots only: an inactive lane holds no values.)
 struct Ops {
-  float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f, q3 = 0.0f, q4 = 0.0f, k0 = 0.0f, k1 = 0.0f;
-  float qw = 0.0f, qh = 0.0f, qr = 0.0f;
+  float q0, q1, q2, q3, q4, k0, k1;
+  float qw, qh, qr;
   bool axial = false;
 };
 
@@ -124,21 +125,18 @@ template <int PHYS, bool RECT>
 __device__ __forceinline__ void load_ops(const ForceArgs& A, uint32_t b, uint32_t slot, Ops& o) {
   constexpr int C = Shape<PHYS>::C;
   constexpr bool AXI = Shape<PHYS>::AXI;
-  o.q0 = ld(A.m[0], b, slot);
-  o.q3 = ld(A.m[3], b, slot);
-  if constexpr (!AXI) o.q4 = ld(A.m[4], b, slot);
-  if constexpr (!RECT) {
-    o.q1 = ld(A.m[1], b, slot);
-    o.q2 = ld(A.m[2], b, slot);
-  }
-  o.k0 = ld(A.c0, b, slot);
-  if constexpr (C >= 2) o.k1 = ld(A.c1, b, slot);
-  if constexpr (AXI) {
-    o.qw = ld(A.wr, b, slot);
-    o.qr = ld(A.r, b, slot);
-    if constexpr (PHYS != kAcousticAxi0) o.qh = ld(A.hoop, b, slot);
-    o.axial = *at32(A.axial, b) != 0;
-  }
+  const uint32_t b4 = b * 4u, s4 = slot * 4u;
+  o.q0 = ld(A.m[0], b4, s4);
+  o.q3 = ld(A.m[3], b4, s4);
+  o.q4 = AXI ? 0.0f : ld(A.m[4], b4, s4);
+  o.q1 = RECT ? 0.0f : ld(A.m[1], b4, s4);
+  o.q2 = RECT ? 0.0f : ld(A.m[2], b4, s4);
+  o.k0 = ld(A.c0, b4, s4);
+  o.k1 = C >= 2 ? ld(A.c1, b4, s4) : 0.0f;
+  o.qw = AXI ? ld(A.wr, b4, s4) : 0.0f;
+  o.qr = AXI ? ld(A.r, b4, s4) : 0.0f;
+  o.qh = AXI && PHYS != kAcousticAxi0 ? ld(A.hoop, b4, s4) : 0.0f;
+  if constexpr (AXI) o.axial = *at32(A.axial, b) != 0;
 }
 
 // Derivative-matrix rows (for the gradient) and columns (for the weak
@@ -449,21 +447,22 @@ __global__ __launch_bounds__(kWG) void k_forces_local(ForceArgs A) {
 // ------------------------------------------------------------ production
 
 // Registers of one element slot.
+// (m, o, code, upd, uu, pv, pim are set for active slots only.)
 template <int C>
 struct Slot {
   bool act = false;
   int b = 0;
-  int32_t m = -1;  // global point (< 0: constrained slot of row -1 - m)
+  int bm = -1;  // the batch's one model (batches of one model), uniform; else -1
+  int32_t m;  // global point (< 0: constrained slot of row -1 - m)
   Ops o;
-  int code = 0;  // slot code (grouped / batch-local)
-  bool upd = false;  // the slot updates its point (owned point or batch-local leader)
+  int code;  // slot code (grouped / batch-local)
+  bool upd;  // the slot updates its point (owned point or batch-local leader)
   float uu[C];
   float pv[C];
-  float pim = 0.0f;
+  float pim;
 };
 
-// First round trip: the slot's point, operands and rank.  In a time step
-// of a batch, elements of models past their last step are skipped.
+// First round trip: the slot's point, operands and rank.
 template <int NG, int PHYS, bool RECT>
 __device__ __forceinline__ void stage_point(const ForceArgs& A, const StepParams& P, int step, int batch, int le,
                                             int k, Slot<Shape<PHYS>::C>& s) {
@@ -471,7 +470,6 @@ __device__ __forceinline__ void stage_point(const ForceArgs& A, const StepParams
   constexpr int EPB = kWG / NPE;
   const int b = batch * EPB + le;
   s.act = le < EPB && b < A.nb;
-  if (s.act && P.fuse && A.models) s.act = step <= A.models[A.emodel[b]].n_steps;
   s.b = b;
   if (s.act) {
     const uint32_t slot = b * NPE + k;
@@ -521,13 +519,18 @@ __device__ __forceinline__ void point_update(const ForceArgs& A, const StepParam
   float rdt = P.rdt, hdt = P.hdt, hdt2 = P.hdt2;
   double cc = P.cc;
   int pred = P.pred;
-  if (A.models) {
-    const ModelCoef& mc = A.models[A.emodel[s.b]];
+  auto take = [&](const ModelCoef& mc) {
     rdt = mc.rdt;
     hdt = mc.hdt;
     hdt2 = mc.hdt2;
     cc = mc.cc;
     pred = pred && step < mc.n_steps;
+  };
+  if (A.models) {
+    if (s.bm >= 0)
+      take(A.models[s.bm]);  // uniform: scalar loads
+    else
+      take(A.models[A.emodel[s.b]]);
   }
   float u0[C], vc[C], ac[C];
 #pragma unroll
@@ -595,9 +598,22 @@ __device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams
                                             int batch, bool energy, double& ep, double& ek) {
   constexpr int NPE = NG * NG;
   constexpr int C = Shape<PHYS>::C;
-  // On a batch, a batch whose models are all past their last step does
-  // nothing (the whole work-group leaves).
-  if (P.fuse && A.batch_last && batch * (kWG / (NG * NG)) < A.nb && step > A.batch_last[batch]) return;
+  // In a time step of a batch: the batch's last step and its one model
+  // (-1: elements of several models), loaded as vector loads (the index
+  // passes through a vector register) so that they are issued and waited
+  // for with the slots' first loads rather than holding the work-group's
+  // first scalar wait.  A batch whose models are all past their last step
+  // leaves before it stores anything; a batch of one model (the usual case)
+  // takes its model's coefficients as uniform values; a batch of several
+  // models checks each element's model.
+  const bool in_batch = P.fuse && A.batch_last && batch * (kWG / (NG * NG)) < A.nb;
+  int last_v = 0, bm_v = -1;
+  if (in_batch) {
+    int bi;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(bi) : "s"(batch));
+    last_v = A.batch_last[bi];
+    bm_v = A.batch_model[bi];
+  }
   const int t = threadIdx.x;
   const int le = t / NPE;
   const int k = t - le * NPE;
@@ -605,6 +621,11 @@ __device__ __forceinline__ void force_batch(const ForceArgs& A, const StepParams
   stage_matrices(A, T);
   Slot<C> s;
   stage_point<NG, PHYS, RECT>(A, P, step, batch, le, k, s);
+  if (in_batch) {
+    if (step > __builtin_amdgcn_readfirstlane(last_v)) return;
+    s.bm = __builtin_amdgcn_readfirstlane(bm_v);
+    if (s.act && s.bm < 0) s.act = step <= A.models[A.emodel[s.b]].n_steps;
+  }
   stage_gather<C>(A, P.fuse, s);
   float* U = T.u + le * C * NPE;
   if (s.act) {
