@@ -46,6 +46,11 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_forces_pair_layered_800x400.txt")
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes(image, hip, ctx, check):
     """Bytes per step, and per force launch of each block.
 
@@ -151,6 +156,7 @@ def batched_leg(n_models, duration, plain_sample, seed=2026):
         s.close()
         plain_done.append(time.perf_counter() - t0)
     plain_s = time.perf_counter() - t0
+    progress(f"batched leg: {n_plain} plain solves in {plain_s:.1f} s")
 
     sims = draw()
     orders = sorted({sim.discretization.order for sim in sims})
@@ -183,6 +189,7 @@ def batched_leg(n_models, duration, plain_sample, seed=2026):
                 b.close()
                 for i in idx:
                     batch_done[i] = time.perf_counter() - t0
+                progress(f"batched leg: a batch of {len(idx)} models done at {batch_done[idx[0]]:.1f} s")
             live = [g for g in live if g[1]._handle is not None]
             if not fed:
                 time.sleep(5e-4)
@@ -245,9 +252,11 @@ def served_leg(n_requests, rate, sweep, seed=7):
     for r in sorted(set(sweep) | {rate}):
         res = dict(rate=r, **run(r, 4, 8))
         out["sweep"].append(res)
+        progress(f"served leg: {res['solves_per_s']:.3f} solves/s at {r:g}/s")
         if r == rate:
             out["served"] = res
     out["one_at_a_time"] = dict(rate=rate, **run(rate, 1, 1))
+    progress(f"served leg: one at a time {out['one_at_a_time']['solves_per_s']:.3f} solves/s")
     out["saturation_solves_per_s"] = max(x["solves_per_s"] for x in out["sweep"])
     out["sample"] = (f"{n_requests} requests, Poisson arrivals (seed {seed}) at rates {sorted(set(sweep) | {rate})}/s: "
                      f"{int((kinds == 'cased').sum())} cased-hole, {int((kinds == 'open').sum())} open-hole, "
@@ -377,11 +386,15 @@ def main():
     }
     dt = solver.dt
     solver.close()  # free the model's device memory before the other legs
+    progress(f"layered model: {result['ms_per_step']:.4f} ms/step")
     if args.batch_models > 0:
         result["batched"] = batched_leg(args.batch_models, args.batch_duration, args.batch_plain_sample)
+        progress(f"batched leg: {result['batched']['value']:.3f} solves/s")
     if args.serve_requests > 0:
         sweep = [float(x) for x in args.serve_sweep.split(",") if x]
         result["served"] = served_leg(args.serve_requests, args.serve_rate, sweep)
+    if not args.no_cpu:
+        progress("CPU oracle leg")
     result["cpu_baseline"] = None if args.no_cpu else cpu_baseline(sim, dt)
     print(json.dumps(result), flush=True)
 
