@@ -131,8 +131,8 @@ def batched_leg(n_models, duration, plain_sample, seed=2026):
     """Cased-hole dipole models (azimuthal order 1) drawn from the batched
     workload's distribution, solved end to end (host setup, power-iteration
     Δt, all steps, traces): all n_models as batches (one per polynomial
-    order) on the GPU, each batch set up while the batches already set up
-    run; and the first plain_sample of the same draw one after another
+    order) on the GPU, the models' host setups built on worker threads
+    while the batches already created run; and the first plain_sample of the same draw one after another
     on the plain path, whose solves/s stands for sequential service of the
     whole set.  Every solve is queued at t = 0; a solve's latency is the
     time until its traces are ready."""
@@ -163,36 +163,36 @@ def batched_leg(n_models, duration, plain_sample, seed=2026):
     t0 = time.perf_counter()
     batch_done = [0.0] * n_models
     batch_steps = 0
+    # The models' host setups (no device work) are built on worker threads
+    # from the start; each batch (one per order) is created on this thread
+    # once its setups are ready, its first steps queued before the next one
+    # is created, and the batches then run side by side, each on its own
+    # stream, fed 512 steps at a time in turn (the device queues hold the
+    # host back when they are full); a batch's traces are read once its last
+    # step has finished.
     live = []
-    # one batch per order, built on a worker thread (host setup and the
-    # models' power iterations) while the batches already built step on
-    # their own streams, fed 512 steps at a time in turn (the device queues
-    # hold the host back when they are full); a batch's traces are read once
-    # its last step has finished
-    with ThreadPoolExecutor(max_workers=1) as builder:
-        building = [[i for i, sim in enumerate(sims) if sim.discretization.order == order] for order in orders]
-        building = [(idx, builder.submit(insem.BatchSolver, [sims[i] for i in idx])) for idx in building]
-        while building or live:
-            while building and building[0][1].done():
-                idx, f = building.pop(0)
-                b = f.result()
-                batch_steps += sum(b.n_steps(m) for m in range(b.n_models))
-                live.append((idx, b))
-            fed = False
-            for idx, b in live:
-                if b.step < b.max_steps:
-                    b.advance(512)
-                    fed = True
-            for idx, b in [g for g in live if g[1].step >= g[1].max_steps and g[1].idle()]:
-                for m in range(b.n_models):
-                    _ = b.traces(m)
-                b.close()
-                for i in idx:
-                    batch_done[i] = time.perf_counter() - t0
-                progress(f"batched leg: a batch of {len(idx)} models done at {batch_done[idx[0]]:.1f} s")
-            live = [g for g in live if g[1]._handle is not None]
-            if not fed:
-                time.sleep(5e-4)
+    with ThreadPoolExecutor(max_workers=8) as pool:
+        images = [pool.submit(sim.build_setup) for sim in sims]
+        for order in orders:
+            idx = [i for i, sim in enumerate(sims) if sim.discretization.order == order]
+            b = insem.BatchSolver([images[i].result() for i in idx])
+            batch_steps += sum(b.n_steps(m) for m in range(b.n_models))
+            b.advance(2048)
+            live.append((idx, b))
+    while live:
+        for idx, b in live:
+            if b.step < b.max_steps:
+                b.advance(512)
+        for idx, b in [g for g in live if g[1].step >= g[1].max_steps and g[1].idle()]:
+            for m in range(b.n_models):
+                _ = b.traces(m)
+            b.close()
+            for i in idx:
+                batch_done[i] = time.perf_counter() - t0
+            progress(f"batched leg: a batch of {len(idx)} models done at {batch_done[idx[0]]:.1f} s")
+        live = [g for g in live if g[1]._handle is not None]
+        if live and all(g[1].step >= g[1].max_steps for g in live):
+            time.sleep(5e-4)
     batch_s = time.perf_counter() - t0
 
     def pct(v, q):

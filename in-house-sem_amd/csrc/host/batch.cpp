@@ -1,10 +1,7 @@
 #include "batch.hpp"
 
 #include <algorithm>
-#include <atomic>
-#include <exception>
 #include <string>
-#include <thread>
 
 #include "error.hpp"
 
@@ -179,42 +176,31 @@ BatchSolver::BatchSolver(std::vector<std::unique_ptr<SetupImage>> models) : mode
   // queues the next ones.
   for (const auto& m : models_) INSEM_REQUIRE(m, "null model in a batch");
   // Each model's own power iteration on a short-lived context of it alone,
-  // several models at once on host threads (building a context is host
-  // work; the iterations of different models overlap on the device).  Every
-  // thread uses the caller's device; contexts are independent, and the
-  // results do not depend on which thread ran which model.
-  const std::size_t n = models_.size();
-  time_.assign(n, ModelTime{});
-  int32_t device = 0;
-  check_hip(insem_hip_get_device(&device));
-  const std::size_t workers =
-      std::min<std::size_t>(n, std::clamp<unsigned>(std::thread::hardware_concurrency(), 1u, 8u));
-  std::atomic<std::size_t> next{0};
-  std::vector<std::exception_ptr> failure(workers);
-  auto work = [&](std::size_t w) {
+  // all on the calling thread.  A window of them is queued before the first
+  // result is read, so that the iterations of different models overlap on
+  // the device while the host builds the next contexts.
+  constexpr std::size_t kWindow = 16;
+  for (std::size_t w0 = 0; w0 < models_.size(); w0 += kWindow) {
+    const std::size_t w1 = std::min(models_.size(), w0 + kWindow);
+    std::vector<insem_hip_ctx*> ctx(w1 - w0, nullptr);
     try {
-      check_hip(insem_hip_set_device(device));
-      for (std::size_t i = next++; i < n; i = next++) {
-        insem_hip_ctx* ctx = nullptr;
-        check_hip(insem_hip_create(&models_[i]->view, &ctx));
-        std::unique_ptr<insem_hip_ctx, int (*)(insem_hip_ctx*)> own(ctx, insem_hip_destroy);
+      for (std::size_t i = w0; i < w1; ++i) {
+        check_hip(insem_hip_create(&models_[i]->view, &ctx[i - w0]));
         const std::vector<float> start = models_[i]->power_start_vectors();
+        check_hip(insem_hip_lambda_start(ctx[i - w0], start.data()));
+      }
+      for (std::size_t i = w0; i < w1; ++i) {
         double lambda = 0.0;
-        check_hip(insem_hip_lambda_max(ctx, start.data(), &lambda));
-        time_[i] = model_time(models_[i]->time, lambda);
+        check_hip(insem_hip_lambda_result(ctx[i - w0], &lambda));
+        time_.push_back(model_time(models_[i]->time, lambda));
+        max_steps_ = std::max(max_steps_, time_.back().n_steps);
       }
     } catch (...) {
-      failure[w] = std::current_exception();
-      next = n;  // the other workers stop after their current model
+      for (insem_hip_ctx* c : ctx) insem_hip_destroy(c);
+      throw;
     }
-  };
-  std::vector<std::thread> pool;
-  for (std::size_t w = 1; w < workers; ++w) pool.emplace_back(work, w);
-  work(0);
-  for (auto& t : pool) t.join();
-  for (const auto& f : failure)
-    if (f) std::rethrow_exception(f);
-  for (const ModelTime& t : time_) max_steps_ = std::max(max_steps_, t.n_steps);
+    for (insem_hip_ctx* c : ctx) insem_hip_destroy(c);
+  }
   std::vector<const SetupImage*> ptrs;
   for (const auto& m : models_) ptrs.push_back(m.get());
   union_ = union_of(ptrs);
