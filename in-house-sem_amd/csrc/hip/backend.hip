@@ -1390,8 +1390,10 @@ int insem_hip_set_time_batch(insem_hip_ctx* c, const double* dt, const int32_t* 
   }
   return guarded(c, [&] {
     c->h2d(c->d_models, mc.data(), mc.size() * sizeof(ModelCoef));
-    // Last step of each force batch and of each 256 points, so that work on
-    // models past their last step is skipped by whole work-groups.
+    // Last step and model of each force batch, so that work on models past
+    // their last step is skipped by whole work-groups and one-model batches
+    // read their coefficients uniformly; the model of each 256 points, for
+    // the tails' uniform coefficients.
     const int epb = force_batch_elements(c->ngll);
     for (DomainRT& R : c->doms) {
       for (BlockRT& br : R.blocks) {
@@ -1409,10 +1411,15 @@ int insem_hip_set_time_batch(insem_hip_ctx* c, const double* dt, const int32_t* 
           br.args.batch_model = c->upload(model);
         }
       }
-      std::vector<int32_t> last(static_cast<size_t>((R.ng + 255) / 256), -1);
-      for (int64_t g = 0; g < R.ng; ++g) last[g >> 8] = std::max(last[g >> 8], n_steps[R.h_pmodel[g]]);
-      if (!R.dd.chunk_last) R.dd.chunk_last = c->upload(last);
-      else c->h2d(const_cast<int32_t*>(R.dd.chunk_last), last.data(), last.size() * sizeof(int32_t));
+      if (!R.dd.chunk_model) {
+        // the one model of each 256 points (-1: points of several models)
+        std::vector<int32_t> model(static_cast<size_t>((R.ng + 255) / 256), -2);
+        for (int64_t g = 0; g < R.ng; ++g) {
+          int32_t& x = model[g >> 8];
+          x = x == -2 || x == R.h_pmodel[g] ? R.h_pmodel[g] : -1;
+        }
+        R.dd.chunk_model = c->upload(model);
+      }
     }
     apply_time(c, dt[0], max_n, amplitudes);
   });

@@ -218,7 +218,7 @@ struct DomDev {
   // single model, whose coefficients come with the launch).
   const int32_t* pmodel = nullptr;
   const ModelCoef* models = nullptr;
-  const int32_t* chunk_last = nullptr;  // per 256 points: the last step of their models
+  const int32_t* chunk_model = nullptr;  // per 256 points: their one model, or -1 (points of several models)
 };
 
 // Special-point data of one domain ([0, n_sp) = points n_reg .. ng - 1).

@@ -88,28 +88,35 @@ struct PointStep {
   bool active;
 };
 
-__device__ __forceinline__ PointStep point_step(const DomDev& d, const StepCoef& sc, int pred, int step, int64_t g) {
+__device__ __forceinline__ PointStep point_step(const DomDev& d, const StepCoef& sc, int pred, int step, int64_t g,
+                                                int bm) {
   PointStep p{sc.rdt, sc.hdt, sc.hdt2, sc.cc, pred, true};
   if (d.models) {
-    const ModelCoef& m = d.models[*at32(d.pmodel, static_cast<uint32_t>(g))];
-    p.rdt = m.rdt;
-    p.hdt = m.hdt;
-    p.hdt2 = m.hdt2;
-    p.cc = m.cc;
-    p.active = step <= m.n_steps;
-    p.pred = pred && step < m.n_steps;
+    auto take = [&](const ModelCoef& m) {
+      p.rdt = m.rdt;
+      p.hdt = m.hdt;
+      p.hdt2 = m.hdt2;
+      p.cc = m.cc;
+      p.active = step <= m.n_steps;
+      p.pred = pred && step < m.n_steps;
+    };
+    if (bm >= 0)
+      take(d.models[bm]);  // uniform
+    else
+      take(d.models[*at32(d.pmodel, static_cast<uint32_t>(g))]);
   }
   return p;
 }
 
-// On a batch, whether every model of the tail block's points [g0, g0 + kTB)
-// is past its last step at `step` (the block then does nothing).
-__device__ __forceinline__ bool tail_block_done(const DomDev& d, int64_t g0, int step) {
-  if (!d.chunk_last) return false;
+// On a batch: the one model of the points of the tail block [g0, g0 + kTB)
+// when they share one (the usual case), else -1; uniform over the block.
+// (Points of models past their last step do nothing: PointStep::active.)
+__device__ __forceinline__ int tail_block_model(const DomDev& d, int64_t g0) {
+  if (!d.chunk_model) return -1;
   const int64_t g1 = (g0 + kTB < d.ng ? g0 + kTB : d.ng) - 1;
-  if (g1 < g0) return true;
-  const int32_t l0 = d.chunk_last[g0 >> 8], l1 = d.chunk_last[g1 >> 8];
-  return step > (l0 > l1 ? l0 : l1);
+  if (g1 < g0) return -1;
+  const int32_t m0 = d.chunk_model[g0 >> 8], m1 = d.chunk_model[g1 >> 8];
+  return m0 == m1 ? m0 : -1;
 }
 
 // a = M^-1 f, optional corrector, energy terms and the next predictor at a
@@ -159,8 +166,8 @@ template <int C, bool W>
 __device__ __forceinline__ void regular_body(const DomDev& d, int64_t first, const StepCoef& sc, int corr, int pred,
                                              int step, bool energy, int vb, double& ep, double& ek) {
   const int64_t g0 = first + static_cast<int64_t>(vb) * kTB, g = g0 + threadIdx.x;
-  if (g >= d.n_reg || tail_block_done(d, g0, step)) return;
-  const PointStep ps = point_step(d, sc, pred, step, g);
+  if (g >= d.n_reg) return;
+  const PointStep ps = point_step(d, sc, pred, step, g, tail_block_model(d, g0));
   if (!ps.active) return;
   float f[C];
   gather_sum<C, W>(d, g, f);
@@ -192,8 +199,8 @@ template <int C>
 __device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc, int corr, int pred, int step,
                                              bool energy, int vb, double& ep, double& ek) {
   const int64_t g0 = d.grp.begin + static_cast<int64_t>(vb) * kTB, g = g0 + threadIdx.x;
-  if (g >= d.grp.end || tail_block_done(d, g0, step)) return;
-  const PointStep ps = point_step(d, sc, pred, step, g);
+  if (g >= d.grp.end) return;
+  const PointStep ps = point_step(d, sc, pred, step, g, tail_block_model(d, g0));
   if (!ps.active) return;
   const uint32_t o = static_cast<uint32_t>(g - d.grp.begin);
   const uint32_t z = static_cast<uint32_t>(d.grp.end - d.grp.begin), rows = static_cast<uint32_t>(d.grp.rows);
@@ -249,8 +256,8 @@ struct Accel {
 
 template <int C, bool W>
 __device__ __forceinline__ Accel special_point(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
-                                               int pred, int step, bool energy, int64_t i, double& ep, double& ek,
-                                               Operand ot_in) {
+                                               int pred, int step, bool energy, int64_t i, int bm, double& ep,
+                                               double& ek, Operand ot_in) {
   const uint32_t g = static_cast<uint32_t>(d.n_reg + i), ng = static_cast<uint32_t>(d.ng);
   // The point's loads are issued level by level (first everything indexed
   // by the point, then what those indices select; coupling operands with a
@@ -268,7 +275,7 @@ __device__ __forceinline__ Accel special_point(const DomDev& d, const SpecialDev
   }
   const float im = *at32(d.im, g);
   const double mass = *at32(d.mass, g);
-  const PointStep ps = point_step(d, sc, pred, step, g);
+  const PointStep ps = point_step(d, sc, pred, step, g, bm);
   float cx = 0.0f, cz = 0.0f;
   uint32_t o = 0;
   if (sp.cpl_cx) {
@@ -298,7 +305,13 @@ __device__ __forceinline__ Accel special_point(const DomDev& d, const SpecialDev
     }
   }
   if (!ps.active) return Accel{};
-  const double h_damp = d.models && sc.h_damp > 0.0 ? d.models[*at32(d.pmodel, g)].h_damp : sc.h_damp;
+  double h_damp = sc.h_damp;
+  if (d.models && sc.h_damp > 0.0) {
+    if (bm >= 0)
+      h_damp = d.models[bm].h_damp;  // uniform
+    else
+      h_damp = d.models[*at32(d.pmodel, g)].h_damp;
+  }
   float f[C];
   gather_rows<C, W>(d, t0, t1, f);
 #pragma unroll
@@ -382,8 +395,9 @@ template <int C, bool W>
 __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
                                              int pred, int step, bool energy, int vb, double& ep, double& ek) {
   const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
-  if (i >= sp.n_sp || tail_block_done(d, d.n_reg + static_cast<int64_t>(vb) * kTB, step)) return;
-  special_point<C, W>(d, sp, sc, corr, pred, step, energy, i, ep, ek, Operand{});
+  if (i >= sp.n_sp) return;
+  const int bm = tail_block_model(d, d.n_reg + static_cast<int64_t>(vb) * kTB);
+  special_point<C, W>(d, sp, sc, corr, pred, step, energy, i, bm, ep, ek, Operand{});
 }
 
 // Coupled model, solid special point i: the solid point, then, when it is on
@@ -397,8 +411,10 @@ __device__ __forceinline__ void special_pair_body(const DomDev& ds, const Specia
                                                   const SpecialDev& spf, const StepCoef& sc, int corr, int pred,
                                                   int step, bool energy, int vb, double& ep, double& ek) {
   const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
-  if (i >= sps.n_sp || tail_block_done(ds, ds.n_reg + static_cast<int64_t>(vb) * kTB, step)) return;
-  const Accel as = special_point<CS, WS>(ds, sps, sc, corr, pred, step, energy, i, ep, ek, Operand{});
+  if (i >= sps.n_sp) return;
+  // (a fluid partner belongs to its solid point's model)
+  const int bm = tail_block_model(ds, ds.n_reg + static_cast<int64_t>(vb) * kTB);
+  const Accel as = special_point<CS, WS>(ds, sps, sc, corr, pred, step, energy, i, bm, ep, ek, Operand{});
   if (!as.active) return;
   const int32_t ci = sps.cpl_idx[i];
   if (ci < 0) return;
@@ -408,7 +424,7 @@ __device__ __forceinline__ void special_pair_body(const DomDev& ds, const Specia
   op.v0 = as.a0;
   op.v1 = as.a1;
   op.v2 = as.a2;
-  special_point<CF, WF>(df, spf, sc, corr, pred, step, energy, i_f, ep, ek, op);
+  special_point<CF, WF>(df, spf, sc, corr, pred, step, energy, i_f, bm, ep, ek, op);
 }
 
 // Coupled model, fluid special points off the interface (the interface ones
@@ -418,9 +434,9 @@ __device__ __forceinline__ void special_uncoupled_body(const DomDev& d, const Sp
                                                        int corr, int pred, int step, bool energy, int vb, double& ep,
                                                        double& ek) {
   const int64_t i = static_cast<int64_t>(vb) * kTB + threadIdx.x;
-  if (i >= sp.n_sp || tail_block_done(d, d.n_reg + static_cast<int64_t>(vb) * kTB, step) || sp.cpl_idx[i] >= 0)
-    return;
-  special_point<C, W>(d, sp, sc, corr, pred, step, energy, i, ep, ek, Operand{});
+  if (i >= sp.n_sp || sp.cpl_idx[i] >= 0) return;
+  const int bm = tail_block_model(d, d.n_reg + static_cast<int64_t>(vb) * kTB);
+  special_point<C, W>(d, sp, sc, corr, pred, step, energy, i, bm, ep, ek, Operand{});
 }
 
 // The tail of a step on one domain, one launch: blocks [0, n_sp_blocks) take
@@ -485,7 +501,7 @@ __global__ __launch_bounds__(kTB, tail_waves(CS)) void k_tail_pair(DomDev ds, Sp
 __global__ __launch_bounds__(kTB) void k_predictor(DomDev d, StepCoef sc, StepClock clk) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
   if (i >= d.ncomp * d.ng) return;
-  const PointStep ps = point_step(d, sc, 1, clock_step(clk) + 1, i % d.ng);
+  const PointStep ps = point_step(d, sc, 1, clock_step(clk) + 1, i % d.ng, -1);
   if (!ps.active) return;
   const float vi = d.v[i], ai = d.a[i];
   d.u[i] = d.u[i] + (ps.rdt * vi + ps.hdt2 * ai);
