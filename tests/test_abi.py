@@ -36,6 +36,15 @@ def test_device_count_never_fails():
     assert _lib.hip().insem_hip_device_count() >= 0
 
 
+def test_device_selection_rejects_missing_devices():
+    hip = _lib.hip()
+    n = hip.insem_hip_device_count()
+    assert hip.insem_hip_set_device(-1) == -1
+    assert hip.insem_hip_set_device(n) == -1
+    assert b"no device" in hip.insem_hip_last_error()
+    assert hip.insem_hip_get_device(None) == -1
+
+
 def test_create_rejects_unsupported_order_before_touching_a_device():
     class Setup(ctypes.Structure):
         _fields_ = [("ngll", ctypes.c_int32)]

@@ -80,7 +80,12 @@ def compare_runs(sim, steps=None, state=None):
 
 
 def test_device_visible():
-    assert _lib.hip().insem_hip_device_count() >= 1
+    hip = _lib.hip()
+    assert hip.insem_hip_device_count() >= 1
+    dev = ctypes.c_int32(-1)
+    assert hip.insem_hip_get_device(ctypes.byref(dev)) == 0
+    assert 0 <= dev.value < hip.insem_hip_device_count()
+    assert hip.insem_hip_set_device(dev.value) == 0
 
 
 def test_cfg0_stable_dt_matches_oracle():
