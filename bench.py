@@ -127,6 +127,49 @@ def cpu_baseline(sim, dt, max_seconds=20.0):
                       f"({el:.1f} s, OpenMP oracle on {int(threads)} threads, stepping only)"}
 
 
+def plain_leg(sim, workload, steps, warmup):
+    """One model on the plain path, as the headline measures it: DOF-steps/s
+    over `steps` replayed steps after `warmup`, then 200 directly launched
+    steps with HIP events around the force launches; the roofline fractions
+    of the step (the survey's per-step bytes) and of the force launch (its
+    algorithmic bytes)."""
+    from insem_amd import _lib
+
+    solver = sim.solver
+    img = solver.image
+    timing_steps = min(steps, 200)
+    if solver.n_steps < warmup + steps + timing_steps:
+        raise SystemExit(f"{workload}: the model has {solver.n_steps} steps, fewer than this run needs")
+    hip, check = _lib.hip(), _lib.check_hip
+    ctx = ctypes.c_void_p(solver.context)
+    solver.advance(warmup)
+    check(hip.insem_hip_synchronize(ctx))
+    t0 = time.perf_counter()
+    solver.advance(steps)
+    check(hip.insem_hip_synchronize(ctx))
+    el = time.perf_counter() - t0
+    check(hip.insem_hip_set_timing(ctx, 1))
+    fm, fn, um, un = ctypes.c_double(), ctypes.c_int32(), ctypes.c_double(), ctypes.c_int32()
+    check(hip.insem_hip_kernel_time(ctx, ctypes.byref(fm), ctypes.byref(fn), ctypes.byref(um), ctypes.byref(un)))
+    solver.advance(timing_steps)
+    check(hip.insem_hip_synchronize(ctx))
+    check(hip.insem_hip_kernel_time(ctx, ctypes.byref(fm), ctypes.byref(fn), ctypes.byref(um), ctypes.byref(un)))
+    check(hip.insem_hip_set_timing(ctx, 0))
+    step_bytes, blocks = algorithmic_bytes(img, hip, ctx, check)
+    ms = 1e3 * el / steps
+    force_ms = fm.value / timing_steps  # all force launches of a step
+    force_bytes = sum(b["bytes"] for b in blocks)
+    out = {"workload": workload, "n_dofs": img.n_dofs, "n_elements": sum(d["n_elements"] for d in img.domains),
+           "ms_per_step": ms, "value": img.n_dofs * steps / el, "unit": "DOF-steps/s",
+           "step_algorithmic_bytes": step_bytes, "step_frac": step_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+           "force_ms_per_step": force_ms, "force_launches_per_step": fn.value / timing_steps,
+           "force_algorithmic_bytes": force_bytes,
+           "force_frac": force_bytes / (force_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+           "tail_ms_per_step": um.value / timing_steps}
+    solver.close()
+    return out
+
+
 def batched_leg(n_models, duration, plain_sample, seed=2026):
     """Cased-hole dipole models (azimuthal order 1) drawn from the batched
     workload's distribution, solved end to end (host setup, power-iteration
@@ -284,6 +327,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--nx", type=int, default=800)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--no-open-hole", action="store_true", help="skip the open-hole monopole leg")
     ap.add_argument("--calibrate-copy", action="store_true", help="run only a 1 GiB device copy (counter scale)")
     ap.add_argument("--batch-models", type=int, default=128, help="models of the batched leg (0: skip it)")
     ap.add_argument("--batch-plain-sample", type=int, default=24,
@@ -387,6 +431,14 @@ def main():
     dt = solver.dt
     solver.close()  # free the model's device memory before the other legs
     progress(f"layered model: {result['ms_per_step']:.4f} ms/step")
+    if not args.no_open_hole:
+        from insem_amd.configs import open_hole_monopole
+
+        result["open_hole"] = plain_leg(
+            open_hole_monopole(), "axisymmetric open-hole monopole, order 4 with GLJ on the axis, 20 kHz AMR, "
+            "formation radius 4 m, 13 receivers", args.steps, args.warmup)
+        progress(f"open-hole model: {result['open_hole']['ms_per_step']:.4f} ms/step, "
+                 f"step at {result['open_hole']['step_frac']:.3f} of the HBM roofline")
     if args.batch_models > 0:
         result["batched"] = batched_leg(args.batch_models, args.batch_duration, args.batch_plain_sample)
         progress(f"batched leg: {result['batched']['value']:.3f} solves/s")

@@ -140,6 +140,27 @@ def cased_hole(rng, duration=5e-3, z=(-1.5, 4.5), formation_radius=4.0, dz=0.08,
     return sim
 
 
+def open_hole_monopole(duration=3e-3):
+    """The axisymmetric open-hole monopole workload (azimuthal order 0): mud
+    column of radius 0.1 m (vp 1500, rho 1000) in a fast formation (vp 4000,
+    vs 2300, rho 2500) out to r = 4 m, z in [-1.5, 4.5] m with dz = 0.08 m,
+    order 4 with GLJ elements on the axis, refined to 6 points per
+    wavelength at 20 kHz (2:1 balanced); absorbing outer, top and bottom
+    edges; an on-axis pressure monopole with an 8 kHz Ricker; 13 pressure
+    receivers at r = 0.05 m, z = 1.0 ... 2.8 m in 0.15 m steps."""
+    bh = mesh.borehole((-1.5, 4.5), [("mud", 0.1)], 4.0, beds=[(-1.5, "formation")], dz=0.08)
+    sim = insem.Simulation(bh.mesh, insem.Discretization(order=4, f_max=20e3, points_per_wavelength=6.0))
+    sim.set_material(bh.regions["mud"], insem.Material.acoustic(1500.0, 1000.0))
+    sim.set_material(bh.regions["formation"], insem.Material.elastic(4000.0, 2300.0, 2500.0))
+    for name in ("bottom", "outer", "top"):
+        sim.set_boundary(bh.boundaries[name], "absorbing")
+    sim.add_source(insem.Source.monopole(0.0, 0.0, insem.ricker(8e3)))
+    for k in range(13):
+        sim.add_receiver(f"r{k}", 0.05, 1.0 + 0.15 * k, ["pressure"])
+    sim.set_time(duration=duration)
+    return sim
+
+
 def open_hole(rng, duration=3e-3, z=(-1.5, 4.5), formation_radius=4.0, dz=0.08, order=None, f_max=None,
               n_receivers=8, receiver_z0=1.0):
     """Axisymmetric open-hole monopole model (azimuthal order 0) drawn from
