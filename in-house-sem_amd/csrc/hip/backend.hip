@@ -89,7 +89,7 @@ struct BlockRT {
 struct DomainRT {
   int medium = 0, ncomp = 1, am = 0, npe = 1;
   int64_t ng = 0, n_int = 0, n_reg = 0, n_sp = 0, ne = 0;
-  int64_t contrib_len = 0;  // floats of the group layout
+  int64_t contrib_len = 0;  // floats of the group rows
   std::vector<int32_t> perm_new, perm_old;
   std::vector<uint8_t> h_dir_mask;  // per old point: 1 Dirichlet, 2 axis (host copy for set_state)
   std::vector<int32_t> h_pmodel;    // batch: model of each point (new numbering)
@@ -99,7 +99,7 @@ struct DomainRT {
   std::vector<BlockRT> blocks;
   float* tmp = nullptr;  // [ncomp][ng] scratch (seams, power iteration)
   float* fl_tmp = nullptr;
-  float* contrib = nullptr;  // group layout of the grouped points' forces
+  float* contrib = nullptr;  // group rows of the grouped points' forces
   int32_t* d_dir = nullptr;
   int n_dir = 0;
   int32_t* d_axis = nullptr;
@@ -868,7 +868,7 @@ void timed_end(insem_hip_ctx* c, hipEvent_t b, bool force) {
 }
 
 // Parameters of the force launches of a time step (owned points updated,
-// grouped forces written in the group layout).
+// grouped forces written in the group rows).
 StepParams fused_params(const StepCoef& sc, int corr, int pred, const StepClock& clk, double* partial) {
   StepParams p;
   p.fuse = 1;

@@ -168,7 +168,7 @@ struct ForceArgs {
   float* a = nullptr;
   const float* im = nullptr;
   const float* mass = nullptr;
-  // Slots of grouped points write their force into the group layout
+  // Slots of grouped points write their force into the group rows
   // instead of f_local (fused steps only).
   const uint16_t* code = nullptr;  // [nb][npe] slot codes (slot_code, local_code)
   int has_local = 0;               // some slot of the block is batch-local

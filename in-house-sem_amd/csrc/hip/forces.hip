@@ -19,7 +19,7 @@
 //
 // In a time step (StepParams::fuse) the slot's force is consumed directly:
 // a point owned by this slot alone is updated in place, a grouped point's
-// slot writes its force into the group layout, and only the remaining
+// slot writes its force into the group rows, and only the remaining
 // slots (special points, constrained slots, other points) write f_local.
 // The step's first force launch carries the previous step's receiver tasks
 // in its first work-groups.
@@ -567,7 +567,7 @@ __device__ __forceinline__ void point_update(const ForceArgs& A, const StepParam
 
 // Stores the slot's force fo where its point's assembly reads it: a
 // batch-local slot in the batch's buffer fob (for the point's leader), a
-// grouped slot in the group layout, any other slot in f_local.  An owned
+// grouped slot in its rank's group row, any other slot in f_local.  An owned
 // point keeps its force for the update (point_update).
 template <int C, int NPE>
 __device__ __forceinline__ void slot_epilogue(const ForceArgs& A, const StepParams& P, const Slot<C>& s, int k,

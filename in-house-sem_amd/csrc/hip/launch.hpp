@@ -35,7 +35,7 @@ struct ReceiverDev {
 // Element forces of one block (gather = read the global field through l2g;
 // otherwise read the element-local field in the reference layout).  With
 // p.fuse (gather only), the points [0, a.n_int) are updated in the
-// kernel, grouped points receive their forces in the group layout and the
+// kernel, grouped points receive their forces in the group rows and the
 // energy partials go to p.partial[work-group][2]; *grid (optional) receives
 // the launch's work-group count.
 hipError_t launch_forces(int ngll, int phys, bool rect, bool gather, const ForceArgs& a, const StepParams& p,
@@ -66,7 +66,7 @@ struct StepCoef {
 
 int tail_blocks(int64_t n);
 // Tail of a step on one domain, one launch: a = M^-1 f from the assembled
-// forces (grouped points from the group layout when `grouped`, the other
+// forces (grouped points from the group rows when `grouped`, the other
 // regular points through the transpose CSR), sources, coupling, Stacey,
 // Dirichlet and axis conditions at the special points (whose pre-predictor
 // values go to the side buffers), optional corrector, energy partials

@@ -68,7 +68,7 @@ def footprint_estimate(img: SetupImage) -> int:
         c, ng, ne = d["ncomp"], d["n_global"], d["n_elements"]
         slots = ne * npe
         total += 4 * (5 * c * ng + 3 * ng)       # u, v, a, f, scratch; mass, 1/m, CSR offsets
-        total += 4 * slots * (1 + 2 * c + c)     # CSR addresses, f_local twice, group layout
+        total += 4 * slots * (1 + 2 * c + 4 * c)  # CSR addresses, f_local twice, group rows (<= 4 c per slot)
         total += slots * 5                       # l2g + rank
         total += 4 * slots * (8 + 2 + 3)         # metric, coefficients, axisymmetric factors
     return int(total * 1.1) + (1 << 20)
