@@ -170,15 +170,17 @@ def plain_leg(sim, workload, steps, warmup):
     return out
 
 
-def batched_leg(n_models, duration, plain_sample, seed=2026):
+def batched_leg(n_models, duration, plain_sample, batch_size=5, seed=2026):
     """Cased-hole dipole models (azimuthal order 1) drawn from the batched
     workload's distribution, solved end to end (host setup, power-iteration
-    Δt, all steps, traces): all n_models as batches (one per polynomial
-    order) on the GPU, the models' host setups built on worker threads
-    while the batches already created run; and the first plain_sample of the same draw one after another
-    on the plain path, whose solves/s stands for sequential service of the
-    whole set.  Every solve is queued at t = 0; a solve's latency is the
-    time until its traces are ready."""
+    Δt, all steps, traces): all n_models on the GPU as batches of up to
+    batch_size models of one polynomial order and similar size, all of them
+    side by side on their own streams (the models' host setups built on
+    worker threads while the batches already created run); and the first
+    plain_sample of the same draw one after another on the plain path, whose
+    solves/s stands for sequential service of the whole set.  Every solve is
+    queued at t = 0; a solve's latency is the time until its traces are
+    ready."""
     import numpy as np
 
     import insem_amd as insem
@@ -205,7 +207,7 @@ def batched_leg(n_models, duration, plain_sample, seed=2026):
     orders = sorted({sim.discretization.order for sim in sims})
     t0 = time.perf_counter()
     batch_done = [0.0] * n_models
-    batch_steps = 0
+    batch_steps = n_batches = 0
     # The models' host setups (no device work) are built on worker threads
     # from the start; each batch (one per order) is created on this thread
     # once its setups are ready, its first steps queued before the next one
@@ -218,10 +220,16 @@ def batched_leg(n_models, duration, plain_sample, seed=2026):
         images = [pool.submit(sim.build_setup) for sim in sims]
         for order in orders:
             idx = [i for i, sim in enumerate(sims) if sim.discretization.order == order]
-            b = insem.BatchSolver([images[i].result() for i in idx])
-            batch_steps += sum(b.n_steps(m) for m in range(b.n_models))
-            b.advance(2048)
-            live.append((idx, b))
+            # batches of up to batch_size models of similar size
+            idx.sort(key=lambda i: sum(d["n_elements"] for d in images[i].result().domains))
+            split = -(-len(idx) // batch_size)
+            n_batches += split
+            for part in range(split):
+                sub = idx[part * len(idx) // split:(part + 1) * len(idx) // split]
+                b = insem.BatchSolver([images[i].result() for i in sub])
+                batch_steps += sum(b.n_steps(m) for m in range(b.n_models))
+                b.advance(2048)
+                live.append((sub, b))
     while live:
         for idx, b in live:
             if b.step < b.max_steps:
@@ -247,12 +255,13 @@ def batched_leg(n_models, duration, plain_sample, seed=2026):
             "latency_ms": {"batched_p50": pct(batch_done, 50), "batched_p99": pct(batch_done, 99),
                            "plain_sequential_p50": pct(plain_done, 50), "plain_sequential_p99": pct(plain_done, 99)},
             "sample": f"{n_models} cased-hole dipole models (seed {seed}, orders {orders}, {duration * 1e3:g} ms each, "
-                      f"{batch_steps} model-steps) batched, end to end; plain: the first {n_plain} of them "
+                      f"{batch_steps} model-steps) as {n_batches} batches of up to {batch_size} models of one order "
+                      f"and similar size, side by side, end to end; plain: the first {n_plain} of them "
                       f"({steps} model-steps) one after another, end to end (latencies of that sequential queue)",
-            "batches": len(orders)}
+            "batches": n_batches}
 
 
-def served_leg(n_requests, rate, sweep, seed=7):
+def served_leg(n_requests, rate, sweep, slots=4, batch=8, seed=7):
     """A Poisson stream of mixed requests (60 % cased-hole dipole, 30 %
     open-hole monopole at 3-8 ms; 10 % 2D layered coupled at 1-4 M nodes for
     0.5-1 s) served by the continuous-batching loop at each arrival rate of
@@ -293,7 +302,7 @@ def served_leg(n_requests, rate, sweep, seed=7):
 
     out = {"sweep": []}
     for r in sorted(set(sweep) | {rate}):
-        res = dict(rate=r, **run(r, 4, 8))
+        res = dict(rate=r, **run(r, slots, batch))
         out["sweep"].append(res)
         progress(f"served leg: {res['solves_per_s']:.3f} solves/s at {r:g}/s")
         if r == rate:
@@ -303,7 +312,7 @@ def served_leg(n_requests, rate, sweep, seed=7):
     out["saturation_solves_per_s"] = max(x["solves_per_s"] for x in out["sweep"])
     out["sample"] = (f"{n_requests} requests, Poisson arrivals (seed {seed}) at rates {sorted(set(sweep) | {rate})}/s: "
                      f"{int((kinds == 'cased').sum())} cased-hole, {int((kinds == 'open').sum())} open-hole, "
-                     f"{int((kinds == 'layered').sum())} layered; served: 4 slots, batches of up to 8; "
+                     f"{int((kinds == 'layered').sum())} layered; served: {slots} slots, batches of up to {batch}; "
                      f"one at a time at {rate:g}/s")
     return out
 
@@ -333,10 +342,14 @@ def main():
     ap.add_argument("--batch-plain-sample", type=int, default=24,
                     help="models of the batched leg's sequential plain comparison (the first of the same draw)")
     ap.add_argument("--batch-duration", type=float, default=5e-3, help="simulated time of each batched model (s)")
+    ap.add_argument("--batch-size", type=int, default=5,
+                    help="models per batch (each polynomial order's models, sorted by size, in batches of up to this)")
     ap.add_argument("--serve-requests", type=int, default=16, help="requests of the served leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=2.0,
                     help="Poisson arrival rate of the served leg's comparison with one-at-a-time service (1/s)")
     ap.add_argument("--serve-sweep", default="1,2,4", help="arrival rates (1/s) the served loop runs at")
+    ap.add_argument("--serve-slots", type=int, default=4, help="batches the served loop keeps on the device at once")
+    ap.add_argument("--serve-batch", type=int, default=8, help="requests per batch of the served loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every run of this build uses a single MI355X "
@@ -440,11 +453,12 @@ def main():
         progress(f"open-hole model: {result['open_hole']['ms_per_step']:.4f} ms/step, "
                  f"step at {result['open_hole']['step_frac']:.3f} of the HBM roofline")
     if args.batch_models > 0:
-        result["batched"] = batched_leg(args.batch_models, args.batch_duration, args.batch_plain_sample)
+        result["batched"] = batched_leg(args.batch_models, args.batch_duration, args.batch_plain_sample,
+                                        args.batch_size)
         progress(f"batched leg: {result['batched']['value']:.3f} solves/s")
     if args.serve_requests > 0:
         sweep = [float(x) for x in args.serve_sweep.split(",") if x]
-        result["served"] = served_leg(args.serve_requests, args.serve_rate, sweep)
+        result["served"] = served_leg(args.serve_requests, args.serve_rate, sweep, args.serve_slots, args.serve_batch)
     if not args.no_cpu:
         progress("CPU oracle leg")
     result["cpu_baseline"] = None if args.no_cpu else cpu_baseline(sim, dt)
