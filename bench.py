@@ -205,45 +205,23 @@ def batched_leg(n_models, duration, plain_sample, batch_size=5, seed=2026):
 
     sims = draw()
     orders = sorted({sim.discretization.order for sim in sims})
-    t0 = time.perf_counter()
     batch_done = [0.0] * n_models
-    batch_steps = n_batches = 0
-    # The models' host setups (no device work) are built on worker threads
-    # from the start; each batch (one per order) is created on this thread
-    # once its setups are ready, its first steps queued before the next one
-    # is created, and the batches then run side by side, each on its own
-    # stream, fed 512 steps at a time in turn (the device queues hold the
-    # host back when they are full); a batch's traces are read once its last
-    # step has finished.
-    live = []
-    with ThreadPoolExecutor(max_workers=8) as pool:
-        images = [pool.submit(sim.build_setup) for sim in sims]
-        for order in orders:
-            idx = [i for i, sim in enumerate(sims) if sim.discretization.order == order]
-            # batches of up to batch_size models of similar size
-            idx.sort(key=lambda i: sum(d["n_elements"] for d in images[i].result().domains))
-            split = -(-len(idx) // batch_size)
-            n_batches += split
-            for part in range(split):
-                sub = idx[part * len(idx) // split:(part + 1) * len(idx) // split]
-                b = insem.BatchSolver([images[i].result() for i in sub])
-                batch_steps += sum(b.n_steps(m) for m in range(b.n_models))
-                b.advance(2048)
-                live.append((sub, b))
-    while live:
-        for idx, b in live:
-            if b.step < b.max_steps:
-                b.advance(512)
-        for idx, b in [g for g in live if g[1].step >= g[1].max_steps and g[1].idle()]:
-            for m in range(b.n_models):
-                _ = b.traces(m)
-            b.close()
-            for i in idx:
-                batch_done[i] = time.perf_counter() - t0
-            progress(f"batched leg: a batch of {len(idx)} models done at {batch_done[idx[0]]:.1f} s")
-        live = [g for g in live if g[1]._handle is not None]
-        if live and all(g[1].step >= g[1].max_steps for g in live):
-            time.sleep(5e-4)
+    # insem_amd.solve_many: each order's models sorted by size and cut into
+    # batches of up to batch_size, all side by side on their own streams
+    # (host setups on worker threads while the batches already created run)
+    n_batches = sum(-(-sum(1 for s in sims if s.discretization.order == o) // batch_size) for o in orders)
+    t0 = time.perf_counter()
+
+    batch_steps = 0
+
+    def done(i, t, n_steps):
+        nonlocal batch_steps
+        batch_done[i] = t
+        batch_steps += n_steps
+        if sum(1 for x in batch_done if x > 0.0) % 32 == 0:
+            progress(f"batched leg: {sum(1 for x in batch_done if x > 0.0)} models done at {t:.1f} s")
+
+    insem.solve_many(sims, batch_size=batch_size, on_done=done)
     batch_s = time.perf_counter() - t0
 
     def pct(v, q):

@@ -30,6 +30,7 @@ from .core import (
     sampled_stf,
 )
 from . import mesh
+from .many import solve_many
 
 __version__ = "0.1.0"
 
@@ -53,4 +54,5 @@ __all__ = [
     "real_dtype",
     "ricker",
     "sampled_stf",
+    "solve_many",
 ]

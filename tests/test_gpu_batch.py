@@ -68,3 +68,28 @@ def test_coupled_cartesian_batch_equals_alone_and_oracle():
             if np.linalg.norm(ref) > 0:
                 err = np.linalg.norm(tb[key] - ref) / np.linalg.norm(ref)
                 assert err <= TRACE_RTOL, (k, key, err)
+
+
+def test_solve_many_equals_solo_runs():
+    # mixed orders and kinds (open-hole monopole, cased-hole dipole) in
+    # batches of two, all side by side; every model's traces equal its solo
+    # run and come back in the order given
+    from insem_amd.configs import open_hole
+
+    def small_open(seed, order):
+        rng = np.random.default_rng(seed)
+        return open_hole(rng, duration=2.5e-4, z=(-0.4, 0.8), formation_radius=0.6, dz=0.08, order=order, f_max=8e3,
+                         n_receivers=3, receiver_z0=0.2)
+
+    specs = [(small_cased, 21, 4), (small_open, 31, 4), (small_cased, 22, 5), (small_cased, 23, 4),
+             (small_open, 32, 4), (small_cased, 24, 4), (small_cased, 25, 5)]
+    seen = []
+    res = insem.solve_many([f(s, o) for f, s, o in specs], batch_size=2,
+                           on_done=lambda i, t, n: seen.append((i, n)))
+    assert sorted(i for i, _ in seen) == list(range(len(specs)))
+    for (f, s, o), r, (i, n) in zip(specs, res, sorted(seen)):
+        alone = f(s, o).solver
+        assert alone.n_steps == n
+        alone.run()
+        assert_same_traces(alone.traces, r)
+        alone.close()
