@@ -30,7 +30,7 @@ from .core import (
     sampled_stf,
 )
 from . import mesh
-from .many import solve_many
+from .many import plan_batches, solve_many
 
 __version__ = "0.1.0"
 
@@ -53,6 +53,7 @@ __all__ = [
     "mesh",
     "real_dtype",
     "ricker",
+    "plan_batches",
     "sampled_stf",
     "solve_many",
 ]

@@ -25,29 +25,41 @@ from .core import BatchSolver, Simulation
 from .serve import batch_key
 
 
+def plan_batches(keys: Sequence, sizes: Sequence[int], batch_size: int) -> List[List[int]]:
+    """Model indices in batches: models of equal key (those that can share a
+    batch), sorted by size, cut into ceil(n / batch_size) batches of nearly
+    equal count (at most batch_size each); groups in order of their first
+    model, every model in exactly one batch."""
+    if batch_size < 1:
+        raise ValueError("batch_size must be at least 1")
+    groups: Dict = {}
+    for i, k in enumerate(keys):
+        groups.setdefault(k, []).append(i)
+    out = []
+    for idx in groups.values():
+        idx = sorted(idx, key=lambda i: (sizes[i], i))
+        n = -(-len(idx) // batch_size)
+        out += [idx[p * len(idx) // n:(p + 1) * len(idx) // n] for p in range(n)]
+    return out
+
+
 def solve_many(sims: Sequence[Simulation], batch_size: int = 5, chunk: int = 512, workers: int = 8,
                on_done: Optional[Callable[[int, float, int], None]] = None) -> List[Dict[Tuple[str, str], np.ndarray]]:
     """Traces of every model, in the order given.  on_done(i, seconds,
     n_steps) is called when model i's traces are ready (seconds since the
     call; the model's own step count)."""
-    if batch_size < 1:
-        raise ValueError("batch_size must be at least 1")
     t0 = time.perf_counter()
     out: List[Optional[Dict[Tuple[str, str], np.ndarray]]] = [None] * len(sims)
     live = []
     with ThreadPoolExecutor(max_workers=max(1, workers)) as pool:
         images = [pool.submit(sim.build_setup) for sim in sims]
-        groups: Dict[Tuple, List[int]] = {}
-        for i in range(len(sims)):
-            groups.setdefault(batch_key(images[i].result()), []).append(i)
-        for idx in groups.values():
-            idx.sort(key=lambda i: sum(d["n_elements"] for d in images[i].result().domains))
-            n = -(-len(idx) // batch_size)
-            for part in range(n):
-                sub = idx[part * len(idx) // n:(part + 1) * len(idx) // n]
-                b = BatchSolver([images[i].result() for i in sub])
-                b.advance(4 * chunk)
-                live.append((sub, b))
+        imgs = [f.result() for f in images]
+        plan = plan_batches([batch_key(im) for im in imgs],
+                            [sum(d["n_elements"] for d in im.domains) for im in imgs], batch_size)
+        for sub in plan:
+            b = BatchSolver([imgs[i] for i in sub])
+            b.advance(4 * chunk)
+            live.append((sub, b))
     while live:
         for _, b in live:
             if b.step < b.max_steps:
