@@ -127,21 +127,20 @@ def cpu_baseline(sim, dt, max_seconds=20.0):
                       f"({el:.1f} s, OpenMP oracle on {int(threads)} threads, stepping only)"}
 
 
-def plain_leg(sim, workload, steps, warmup):
-    """One model on the plain path, as the headline measures it: DOF-steps/s
-    over `steps` replayed steps after `warmup`, then 200 directly launched
-    steps with HIP events around the force launches; the roofline fractions
-    of the step (the survey's per-step bytes) and of the force launch (its
-    algorithmic bytes)."""
+def measure_plain(solver, steps, warmup, what="the model"):
+    """The plain path's figures of one model: `steps` replayed steps (the
+    production path: step chunks replayed as hipGraphs) timed after
+    `warmup`, then up to 200 further steps launched directly with HIP events
+    around each force launch on the solver's stream, and the algorithmic
+    bytes of the step and of each block's force work."""
     from insem_amd import _lib
 
-    solver = sim.solver
-    img = solver.image
-    timing_steps = min(steps, 200)
-    if solver.n_steps < warmup + steps + timing_steps:
-        raise SystemExit(f"{workload}: the model has {solver.n_steps} steps, fewer than this run needs")
     hip, check = _lib.hip(), _lib.check_hip
     ctx = ctypes.c_void_p(solver.context)
+    timing_steps = min(steps, 200)
+    need = warmup + steps + timing_steps
+    if solver.n_steps < need:
+        raise SystemExit(f"{what} has {solver.n_steps} steps, fewer than the {need} this run needs")
     solver.advance(warmup)
     check(hip.insem_hip_synchronize(ctx))
     t0 = time.perf_counter()
@@ -155,17 +154,29 @@ def plain_leg(sim, workload, steps, warmup):
     check(hip.insem_hip_synchronize(ctx))
     check(hip.insem_hip_kernel_time(ctx, ctypes.byref(fm), ctypes.byref(fn), ctypes.byref(um), ctypes.byref(un)))
     check(hip.insem_hip_set_timing(ctx, 0))
-    step_bytes, blocks = algorithmic_bytes(img, hip, ctx, check)
-    ms = 1e3 * el / steps
-    force_ms = fm.value / timing_steps  # all force launches of a step
-    force_bytes = sum(b["bytes"] for b in blocks)
+    step_bytes, blocks = algorithmic_bytes(solver.image, hip, ctx, check)
+    return {"el": el, "timing_steps": timing_steps, "force_ms": fm.value, "force_n": fn.value, "tail_ms": um.value,
+            "step_bytes": step_bytes, "blocks": blocks, "hip": hip, "ctx": ctx, "check": check}
+
+
+def plain_leg(sim, workload, steps, warmup):
+    """One model on the plain path, measured as the headline is: DOF-steps/s
+    and the roofline fractions of the step (the survey's per-step bytes) and
+    of its force launches (their algorithmic bytes)."""
+    solver = sim.solver
+    img = solver.image
+    m = measure_plain(solver, steps, warmup, workload)
+    ms = 1e3 * m["el"] / steps
+    force_ms = m["force_ms"] / m["timing_steps"]  # all force launches of a step
+    force_bytes = sum(b["bytes"] for b in m["blocks"])
     out = {"workload": workload, "n_dofs": img.n_dofs, "n_elements": sum(d["n_elements"] for d in img.domains),
-           "ms_per_step": ms, "value": img.n_dofs * steps / el, "unit": "DOF-steps/s",
-           "step_algorithmic_bytes": step_bytes, "step_frac": step_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
-           "force_ms_per_step": force_ms, "force_launches_per_step": fn.value / timing_steps,
+           "ms_per_step": ms, "value": img.n_dofs * steps / m["el"], "unit": "DOF-steps/s",
+           "step_algorithmic_bytes": m["step_bytes"],
+           "step_frac": m["step_bytes"] / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+           "force_ms_per_step": force_ms, "force_launches_per_step": m["force_n"] / m["timing_steps"],
            "force_algorithmic_bytes": force_bytes,
            "force_frac": force_bytes / (force_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
-           "tail_ms_per_step": um.value / timing_steps}
+           "tail_ms_per_step": m["tail_ms"] / m["timing_steps"]}
     solver.close()
     return out
 
@@ -338,7 +349,6 @@ def main():
         return
 
     import insem_amd  # noqa: F401
-    from insem_amd import _lib
     from insem_amd.configs import layered_coupled
 
     nx = args.nx
@@ -348,40 +358,16 @@ def main():
     solver = sim.solver
     setup_s = time.perf_counter() - t0
     img = solver.image
-    timing_steps = min(args.steps, 200)
-    need = args.warmup + args.steps + timing_steps
-    if solver.n_steps < need:
-        raise SystemExit(f"the model has {solver.n_steps} steps, fewer than the {need} this run needs")
-    hip = _lib.hip()
-    check = _lib.check_hip
-    ctx = ctypes.c_void_p(solver.context)
-
-    # Timed region: the production path (step chunks replayed as hipGraphs).
-    solver.advance(args.warmup)
-    check(hip.insem_hip_synchronize(ctx))
-    t0 = time.perf_counter()
-    solver.advance(args.steps)
-    check(hip.insem_hip_synchronize(ctx))
-    el = time.perf_counter() - t0
-
-    # Kernel durations: further steps launched directly, each element-force
-    # launch bracketed by HIP events on the solver's stream.
-    check(hip.insem_hip_set_timing(ctx, 1))
-    fm, fn, um, un = ctypes.c_double(), ctypes.c_int32(), ctypes.c_double(), ctypes.c_int32()
-    check(hip.insem_hip_kernel_time(ctx, ctypes.byref(fm), ctypes.byref(fn), ctypes.byref(um), ctypes.byref(un)))
-    solver.advance(timing_steps)
-    check(hip.insem_hip_synchronize(ctx))
-    check(hip.insem_hip_kernel_time(ctx, ctypes.byref(fm), ctypes.byref(fn), ctypes.byref(um), ctypes.byref(un)))
-    check(hip.insem_hip_set_timing(ctx, 0))
-
+    m = measure_plain(solver, args.steps, args.warmup)
+    hip, ctx, check = m["hip"], m["ctx"], m["check"]
+    el, timing_steps, step_bytes, blocks = m["el"], m["timing_steps"], m["step_bytes"], m["blocks"]
     dof = img.n_dofs
     value = dof * args.steps / el
-    step_bytes, blocks = algorithmic_bytes(img, hip, ctx, check)
-    launches_per_step = fn.value / timing_steps
+    launches_per_step = m["force_n"] / timing_steps
     if round(launches_per_step) == 1:
         # one force launch per step covers every block of both domains
         kbytes = sum(b["bytes"] for b in blocks)
-        k_ms = fm.value / max(fn.value, 1)
+        k_ms = m["force_ms"] / max(m["force_n"], 1)
         kname = "element forces + owned-point update, all blocks of both domains + previous-step receivers"
         kblocks = blocks
     else:
@@ -415,7 +401,7 @@ def main():
                      "launches_per_step": launches_per_step,
                      "blocks": [{k: b[k] for k in ("medium", "kind", "n_elements", "bytes", "owned_slots",
                                                    "local_slots", "points_read")} for b in kblocks],
-                     "tail_ms_per_step": um.value / timing_steps, "timing_steps": timing_steps,
+                     "tail_ms_per_step": m["tail_ms"] / timing_steps, "timing_steps": timing_steps,
                      "step_algorithmic_bytes": step_bytes,
                      "step_frac": step_bytes / (el / args.steps) / (HBM_PEAK_GBS * 1e9)},
     }
