@@ -313,3 +313,48 @@ def test_layered_coupled_small_parity():
     np.testing.assert_allclose(eg[:, 0], eo[:, 0], rtol=1e-12)
     scale = np.abs(eo[:, 1:]).max()
     np.testing.assert_allclose(eg[:, 1:], eo[:, 1:], atol=1e-5 * scale)
+
+
+# Every polynomial order the kernels are built for (NG = 2 .. 9), on each
+# physics: the element forces bitwise against the oracle, and a short run of
+# the fused time step at two more orders.
+
+@pytest.mark.parametrize("order", [1, 2, 3, 5, 6, 7, 8])
+@pytest.mark.parametrize("azimuthal_order", [0, 1])
+def test_borehole_forces_bitwise_every_order(order, azimuthal_order):
+    sim = small_borehole(azimuthal_order, order)
+    s = sim.solver
+    img = sim.build_setup()
+    assert img.ngll == order + 1
+    o = OracleRun(img, dt=s.dt)
+    rng = np.random.default_rng(order)
+    for di, d in enumerate(img.domains):
+        u = rng.uniform(-1, 1, size=(d["ncomp"], d["n_global"])).astype(np.float32)
+        assert np.array_equal(hip_internal_forces(s, di, u)[0], o.internal_forces(di, u)[0]), (order, di)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 5, 6, 7, 8])
+def test_cartesian_forces_bitwise_every_order(order):
+    # sheared (Curved) cells: the general metric path of both media
+    for medium in ("elastic", "acoustic"):
+        sim = insem.Simulation(grid(3, 2, 0.0, 1.5, 0.0, 1.0, 0.3), insem.Discretization(order=order))
+        sim.set_material(0, insem.Material.elastic(3.0, 1.5, 2.0) if medium == "elastic"
+                         else insem.Material.acoustic(1.5, 1.0))
+        sim.set_time(n_steps=4)
+        s = sim.solver
+        img = sim.build_setup()
+        o = OracleRun(img, dt=s.dt)
+        d = img.domains[0]
+        rng = np.random.default_rng(10 + order)
+        u = rng.uniform(-1, 1, size=(d["ncomp"], d["n_global"])).astype(np.float32)
+        assert np.array_equal(hip_internal_forces(s, 0, u)[0], o.internal_forces(0, u)[0]), (order, medium)
+        s.close()
+
+
+@pytest.mark.parametrize("order", [2, 7])
+def test_borehole_step_parity_other_orders(order):
+    sim = small_borehole(1, order)
+    sim.set_time(n_steps=120, energy_every=40)
+    worst, s, o, img = compare_runs(sim)
+    assert s.step == s.n_steps == 120
+    assert worst <= TRACE_RTOL, worst
