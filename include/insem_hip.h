@@ -182,8 +182,7 @@ const char* insem_hip_last_error(void);
 int insem_hip_device_count(void);
 /* The calling thread's current device, and setting it (contexts are created
  * on the current device).  No counterpart in the reference, which has no
- * device; a host that sets models up on several threads makes every thread
- * use the caller's device with these. */
+ * device. */
 int insem_hip_get_device(int32_t* device);
 int insem_hip_set_device(int32_t device);
 
