@@ -161,7 +161,7 @@ def measure_plain(solver, steps, warmup, what="the model"):
 
 def plain_leg(sim, workload, steps, warmup):
     """One model on the plain path, measured as the headline is: DOF-steps/s
-    and the roofline fractions of the step (the survey's per-step bytes) and
+    and the roofline fractions of the step (its per-step algorithmic bytes) and
     of its force launches (their algorithmic bytes)."""
     solver = sim.solver
     img = solver.image
