@@ -1036,11 +1036,7 @@ void chunk_launches(insem_hip_ctx* c, int k, bool first, bool last) {
 // Runs a chunk, replaying a captured hipGraph of it when graphs are on (the
 // step numbers come from the device clock, so one graph serves every chunk
 // of the same shape).  Timing runs launch directly, with events.
-void run_chunk(insem_hip_ctx* c, int k, bool first, bool last) {
-  if (!c->use_graphs || c->timing) {
-    chunk_launches(c, k, first, last);
-    return;
-  }
+hipGraphExec_t chunk_graph(insem_hip_ctx* c, int k, bool first, bool last) {
   const int key = (k << 2) | (first ? 2 : 0) | (last ? 1 : 0);
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
@@ -1061,8 +1057,30 @@ void run_chunk(insem_hip_ctx* c, int k, bool first, bool last) {
     (void)hipGraphDestroy(g);
     HIP_TRY(e);
     it = c->graphs.emplace(key, ge).first;
+    // Upload the graph's resources now, so that its first replay only
+    // launches.
+    HIP_TRY(hipGraphUpload(ge, c->stream));
   }
-  HIP_TRY(hipGraphLaunch(it->second, c->stream));
+  return it->second;
+}
+
+void run_chunk(insem_hip_ctx* c, int k, bool first, bool last) {
+  if (!c->use_graphs || c->timing) {
+    chunk_launches(c, k, first, last);
+    return;
+  }
+  HIP_TRY(hipGraphLaunch(chunk_graph(c, k, first, last), c->stream));
+}
+
+// The chunks of an advance of n steps from the current step: (k, first, last)
+// for each, in order.
+template <class F>
+void for_each_chunk(const insem_hip_ctx* c, int n, F&& f) {
+  const int m = std::max(0, std::min(n, c->n_steps - c->step));
+  for (int s = 0; s < m; s += c->chunk) {
+    const int k = std::min(c->chunk, m - s);
+    f(k, s == 0, s + k == m);
+  }
 }
 
 void require_time(insem_hip_ctx* c) {
@@ -1463,11 +1481,16 @@ int insem_hip_advance(insem_hip_ctx* c, int32_t n) {
     require_time(c);
     if (!c->initialized) initialize(c);
     const int m = std::max(0, std::min(n, c->n_steps - c->step));
-    for (int s = 0; s < m; s += c->chunk) {
-      const int k = std::min(c->chunk, m - s);
-      run_chunk(c, k, s == 0, s + k == m);
-    }
+    for_each_chunk(c, m, [&](int k, bool first, bool last) { run_chunk(c, k, first, last); });
     c->step += m;
+  });
+}
+
+int insem_hip_prepare(insem_hip_ctx* c, int32_t n) {
+  return guarded(c, [&] {
+    require_time(c);
+    if (!c->use_graphs) return;
+    for_each_chunk(c, n, [&](int k, bool first, bool last) { (void)chunk_graph(c, k, first, last); });
   });
 }
 

@@ -227,6 +227,13 @@ int insem_hip_get_field(insem_hip_ctx* ctx, int32_t domain, int32_t field, float
  * solver.cpp:917-963, with initialize 878-898 on first use). */
 int insem_hip_advance(insem_hip_ctx* ctx, int32_t n);
 
+/* Captures and instantiates, without running them, the step-chunk graphs that
+ * insem_hip_advance(ctx, n) would replay from the current step, so that a
+ * later advance of that length only launches.  No counterpart in the
+ * reference (which has no graphs); a timing harness calls it before its
+ * timed region, and a serving loop before its first chunk. */
+int insem_hip_prepare(insem_hip_ctx* ctx, int32_t n);
+
 /* Steps done. */
 int insem_hip_step(insem_hip_ctx* ctx, int32_t* step);
 
