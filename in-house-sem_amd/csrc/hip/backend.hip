@@ -97,7 +97,7 @@ struct DomainRT {
   DomDev dd;
   SpecialDev sp;
   std::vector<BlockRT> blocks;
-  float* tmp = nullptr;  // [ncomp][ng] scratch (seams, power iteration)
+  float* tmp = nullptr;  // [ncomp][ng] work field (seams, power iteration)
   float* fl_tmp = nullptr;
   float* contrib = nullptr;  // group rows of the grouped points' forces
   int32_t* d_dir = nullptr;
@@ -159,7 +159,7 @@ struct insem_hip_ctx {
   int partial_rows = 0;
   double* energy_rows = nullptr;  // [row][2]: sum u.f, kinetic
   int max_energy_rows = 0;
-  double* scratch = nullptr;  // small reduction outputs
+  double* red_out = nullptr;  // small reduction outputs
   // timing
   int timing = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_force, ev_update;
@@ -1326,7 +1326,7 @@ int create_impl(const insem_hip_setup* s0, const insem_hip_batch* batch, insem_h
     int dot_rows = 0;
     for (const DomainRT& R : c->doms) dot_rows = std::max(dot_rows, tail_blocks(R.ncomp * R.ng));
     c->partial = c->alloc<double>(3 * static_cast<size_t>(std::max(rows, dot_rows)) + 16);
-    c->scratch = c->alloc<double>(128);
+    c->red_out = c->alloc<double>(128);
     c->d_clock = c->alloc<int32_t>(1);
     c->zero(c->d_clock, sizeof(int32_t));
     if (const char* g = std::getenv("INSEM_HIP_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
@@ -1565,8 +1565,8 @@ int insem_hip_internal_forces(insem_hip_ctx* c, int32_t di, const float* u, floa
     if (dot) {
       int nb = 0;
       HIP_TRY(launch_dot(R.tmp, R.dd.f, static_cast<int64_t>(R.ncomp) * R.ng, c->partial, &nb, c->stream));
-      HIP_TRY(launch_reduce(c->partial, nb, 1, c->scratch, 0, c->stream));
-      HIP_TRY(hipMemcpyAsync(dot, c->scratch, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(launch_reduce(c->partial, nb, 1, c->red_out, 0, c->stream));
+      HIP_TRY(hipMemcpyAsync(dot, c->red_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
     HIP_TRY(hipMemcpyAsync(h.data(), R.dd.f, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1600,6 +1600,66 @@ int insem_hip_block_forces(insem_hip_ctx* c, int32_t di, int32_t bi, const float
   });
 }
 
+// Output staging of insem_hip_block_forces_noalloc: one array per GPU, held
+// from the library's load until the process ends.  Host code uses only the
+// device address hipGetSymbolAddress returns for it on each device (the
+// array's name is a host-side stand-in, not a device address).
+constexpr size_t kNoAllocFloats = size_t{8} << 20;  // 32 MiB
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call on that device.
+float* scratch_address(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) {
+    set_error(INSEM_HIP_EDEVICE, "device %d is beyond the %d devices the no-allocation path supports", device,
+              kMaxScratchDevices);
+    throw HipFailure{INSEM_HIP_EDEVICE};
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<float*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
+}  // namespace
+
+int insem_hip_block_forces_noalloc(insem_hip_ctx* c, int32_t di, int32_t bi, const float* ul, float* fl) {
+  if (!c || di < 0 || di >= static_cast<int>(c->doms.size())) return set_error(INSEM_HIP_EINVAL, "invalid domain");
+  if (bi < 0 || bi >= static_cast<int>(c->doms[di].blocks.size())) return set_error(INSEM_HIP_EINVAL, "invalid block");
+  if (!ul || !fl) return set_error(INSEM_HIP_EINVAL, "null field");
+  const DomainRT& R0 = c->doms[di];
+  const size_t n = static_cast<size_t>(R0.ne) * R0.ncomp * R0.npe;
+  if (n > kNoAllocFloats)
+    return set_error(INSEM_HIP_EINVAL,
+                     "element-local field of %zu values exceeds the no-allocation limit of %zu values (32 MiB)", n,
+                     kNoAllocFloats);
+  return guarded(c, [&] {
+    DomainRT& R = c->doms[di];
+    BlockRT& br = R.blocks[bi];
+    float* dout = scratch_address(c->device);
+    // The caller's f_local goes in first, so elements outside the block
+    // keep their values whatever an earlier call left in the array.
+    HIP_TRY(hipMemcpyAsync(R.fl_tmp, ul, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(dout, fl, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    ForceArgs a = br.args;
+    a.ul = R.fl_tmp;
+    a.fl = dout;
+    a.out_ref = 1;
+    HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, false, a, StepParams(), c->stream));
+    HIP_TRY(hipMemcpyAsync(fl, dout, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  });
+}
+
 // Power iteration (solver.cpp:583-667) with FP64 reductions on the device.
 int insem_hip_lambda_start(insem_hip_ctx* c, const float* start) {
   if (!start) return set_error(INSEM_HIP_EINVAL, "null argument");
@@ -1607,14 +1667,14 @@ int insem_hip_lambda_start(insem_hip_ctx* c, const float* start) {
   if (c && c->n_models) return set_error(INSEM_HIP_ESTATE, "the power iteration is not available on a batch");
   return guarded(c, [&] {
     hipStream_t st = c->stream;
-    // scratch: per run r (each domain, then the coupled pair) [32 r + 0..2]
+    // red_out: per run r (each domain, then the coupled pair) [32 r + 0..2]
     // = lambda, done, scale; [32 r + 8..10] reductions; [32 r + 16..17] nx, ny
     c->lam_runs = 0;
     c->lam_host.clear();
     size_t off = 0;
     for (size_t di = 0; di < c->doms.size(); ++di) {
       DomainRT& R = c->doms[di];
-      double* state = c->scratch + 32 * c->lam_runs;
+      double* state = c->red_out + 32 * c->lam_runs;
       double* out3 = state + 8;
       const size_t len = static_cast<size_t>(R.ncomp) * R.ng;
       c->lam_host.emplace_back();
@@ -1640,7 +1700,7 @@ int insem_hip_lambda_start(insem_hip_ctx* c, const float* start) {
       DomainRT& S = c->doms[c->solid];
       DomainRT& F = c->doms[c->fluid];
       DomainRT* order[2] = {&S, &F};
-      double* state = c->scratch + 32 * c->lam_runs;
+      double* state = c->red_out + 32 * c->lam_runs;
       double* nxy = state + 16;
       // Keep u and a (device copies: tmp holds u, fl_tmp a); start vectors
       // solid u, then fluid u.
@@ -1706,7 +1766,7 @@ int insem_hip_lambda_result(insem_hip_ctx* c, double* lambda_out) {
   if (c && c->lam_runs == 0) return set_error(INSEM_HIP_ESTATE, "insem_hip_lambda_start has not been called");
   return guarded(c, [&] {
     std::vector<double> st(32 * static_cast<size_t>(c->lam_runs));
-    HIP_TRY(hipMemcpyAsync(st.data(), c->scratch, st.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(st.data(), c->red_out, st.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     double best = 0.0;
     for (int r = 0; r < c->lam_runs; ++r) best = std::max(best, st[32 * static_cast<size_t>(r)]);

@@ -65,6 +65,8 @@ def hip():
         _declare(lib, "insem_hip_read_energy", c_int, c_void, P(c_double))
         _declare(lib, "insem_hip_internal_forces", c_int, c_void, c_int, P(ctypes.c_float), P(ctypes.c_float), P(c_double))
         _declare(lib, "insem_hip_block_forces", c_int, c_void, c_int, c_int, P(ctypes.c_float), P(ctypes.c_float))
+        _declare(lib, "insem_hip_block_forces_noalloc", c_int, c_void, c_int, c_int, P(ctypes.c_float),
+                 P(ctypes.c_float))
         _declare(lib, "insem_hip_synchronize", c_int, c_void)
         _declare(lib, "insem_hip_query", c_int, c_void, P(c_int))
         _declare(lib, "insem_hip_device_memory", c_int, P(c_i64), P(c_i64))

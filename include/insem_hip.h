@@ -263,6 +263,23 @@ int insem_hip_read_energy(insem_hip_ctx* ctx, double* out);
 int insem_hip_internal_forces(insem_hip_ctx* ctx, int32_t domain, const float* u, float* f, double* dot);
 int insem_hip_block_forces(insem_hip_ctx* ctx, int32_t domain, int32_t block, const float* u_local,
                            float* f_local);
+/*
+ * insem_hip_block_forces without any device allocation: no counterpart in
+ * the reference.  It is for callers that cannot allocate (hipMalloc) at call
+ * time: code under graph capture, tight serving loops, harnesses that forbid
+ * allocation.  Same arguments, results and errors as insem_hip_block_forces
+ * (bitwise equal f_local), but its device output staging is a fixed 32 MiB
+ * array per GPU (held from the library's load until the process ends)
+ * instead of a buffer allocated per call.  A domain whose element-local
+ * field exceeds 8,388,608 values (n_elements * ncomp * npe) is refused with
+ * INSEM_HIP_EINVAL.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (from several host threads or contexts) race and
+ * must be serialised by the caller; calls in sequence are independent (the
+ * caller's f_local is copied in first, so nothing of an earlier call
+ * remains in the result).
+ */
+int insem_hip_block_forces_noalloc(insem_hip_ctx* ctx, int32_t domain, int32_t block, const float* u_local,
+                                   float* f_local);
 
 /* Blocks until all work of the context has finished. */
 int insem_hip_synchronize(insem_hip_ctx* ctx);
