@@ -15,9 +15,42 @@
  *   energy, power iteration
  * Each function below names the lines it restates.  OpenMP parallelises the
  * same loops the reference parallelises; results do not depend on the
- * thread count.
+ * thread count.  Built twice: liboracle.so computes in float (the
+ * reference's float build), liboracle64.so in double (ORACLE_REAL=double,
+ * the reference's double build) from the same float setup arrays.
+ *
+ * Pinned, independently of these formulas, by the tests in tests/:
+ *   element kernels, gather/assemble  closed-form exactness, adjointness,
+ *                                     mass = area (test_oracle_operator.py,
+ *                                     test_setup.py)
+ *   sources (force, moment tensor,    2D and 3D (axisymmetric m = 0, 1)
+ *   monopole, dipole), receivers,     Green's functions
+ *   particle velocity, m = 1 rotation (test_oracle_verification.py)
+ *   Stacey (acoustic, elastic P/S,    normal-incidence echoes, monotone decay
+ *   axisymmetric m = 0/1)             (test_oracle_verification.py)
+ *   coupling                          normal-incidence R/T, fluid-solid
+ *                                     reciprocity (Cartesian and m = 1)
+ *   power iteration                   dense eigenvalues, the coupled stability
+ *                                     limit (test_oracle_physics.py,
+ *                                     test_oracle_verification.py)
+ *   energy, time loop                 conservation to round-off, integrator
+ *                                     identity (test_oracle_physics.py,
+ *                                     test_oracle_fp64.py)
+ * Parity unpinned: none of the functions; only the exact value of the
+ * reference's stable_dt (its libstdc++ start vectors, generated by the host
+ * library, not here) is fixed by bounds and the checks above alone.
  */
 #include "insem_oracle.h"
+
+/* Arithmetic and state type: float restates the reference's float build
+ * (INSEM_REAL=float), double its double build (INSEM_REAL=double,
+ * cpp/include/insem/core/config.hpp:19-23).  The setup arrays are float in
+ * both (the double build reads float-rounded coefficients and computes in
+ * double). */
+#ifndef ORACLE_REAL
+#define ORACLE_REAL float
+#endif
+typedef ORACLE_REAL real;
 
 #include <math.h>
 #include <stdarg.h>
@@ -53,7 +86,7 @@ int oracle_set_threads(int32_t n) {
 }
 
 typedef struct {
-  float *u, *v, *a, *f, *ul, *fl;
+  real *u, *v, *a, *f, *ul, *fl;
 } dom_state;
 
 typedef struct {
@@ -65,8 +98,8 @@ struct oracle_ctx {
   dom_state* st;
   double dt;
   int n_steps, step, initialized, have_time;
-  const float* amp; /* [n_sources][n_steps + 1], copied */
-  float* amp_copy;
+  const real* amp; /* [n_sources][n_steps + 1], copied */
+  real* amp_copy;
   double potential;
   /* traces: per receiver-quantity a growable [n_samples][ncomp] buffer */
   int n_traces;
@@ -86,27 +119,36 @@ static int dom_of(const oracle_ctx* c, int medium) {
 
 static size_t dom_len(const insem_hip_domain* d) { return (size_t)d->ncomp * (size_t)d->n_global; }
 
+/* Float values in and out of the state arrays (a copy in the float build). */
+static void from_float(real* dst, const float* src, size_t n) {
+  for (size_t i = 0; i < n; ++i) dst[i] = (real)src[i];
+}
+
+static void to_float(float* dst, const real* src, size_t n) {
+  for (size_t i = 0; i < n; ++i) dst[i] = (float)src[i];
+}
+
 /* ------------------------------------------------------- gather / assemble */
 
 /* dof_map.hpp:94-120 */
-static void gather(const insem_hip_domain* d, int npe, const float* g, float* l) {
+static void gather(const insem_hip_domain* d, int npe, const real* g, real* l) {
   const size_t ng = (size_t)d->n_global;
   const int nc = d->ncomp;
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < d->n_elements; ++e) {
     const int32_t* map = d->l2g + (size_t)e * npe;
     for (int c = 0; c < nc; ++c) {
-      const float* gc = g + c * ng;
-      float* lc = l + ((size_t)e * nc + c) * npe;
+      const real* gc = g + c * ng;
+      real* lc = l + ((size_t)e * nc + c) * npe;
       for (int k = 0; k < npe; ++k) {
         const int32_t m = map[k];
         if (m >= 0) {
           lc[k] = gc[m];
         } else {
           const int32_t r = -1 - m;
-          float s = 0.0f;
+          real s = (real)0;
           for (int32_t q = d->row_offsets[r]; q < d->row_offsets[r + 1]; ++q)
-            s += (float)d->row_weights[q] * gc[d->row_dofs[q]];
+            s += (real)d->row_weights[q] * gc[d->row_dofs[q]];
           lc[k] = s;
         }
       }
@@ -115,20 +157,20 @@ static void gather(const insem_hip_domain* d, int npe, const float* g, float* l)
 }
 
 /* dof_map.hpp:122-141 */
-static void assemble(const insem_hip_domain* d, int npe, const float* l, float* g) {
+static void assemble(const insem_hip_domain* d, int npe, const real* l, real* g) {
   const size_t ng = (size_t)d->n_global;
   const int nc = d->ncomp;
   const int weighted = d->t_weights != NULL;
 #pragma omp parallel for schedule(static)
   for (int gi = 0; gi < d->n_global; ++gi) {
     for (int c = 0; c < nc; ++c) {
-      float sum = 0.0f;
+      real sum = (real)0;
       for (int32_t s = d->t_offsets[gi]; s < d->t_offsets[gi + 1]; ++s) {
         const size_t slot = (size_t)d->t_slots[s];
         const size_t e = slot / (size_t)npe;
         const size_t k = slot - e * (size_t)npe;
-        const float v = l[(e * nc + c) * npe + k];
-        sum += weighted ? (float)d->t_weights[s] * v : v;
+        const real v = l[(e * nc + c) * npe + k];
+        sum += weighted ? (real)d->t_weights[s] * v : v;
       }
       g[c * ng + gi] = sum;
     }
@@ -138,9 +180,9 @@ static void assemble(const insem_hip_domain* d, int npe, const float* l, float* 
 /* ---------------------------------------------------------- element kernels */
 
 /* kernels.cpp:40-51 and 168-179 (Dx = Dy on Cartesian blocks) */
-static inline void ref_deriv(int ng, const float* Dx, const float* Dy, const float* u, int i, int j, float* u_xi,
-                             float* u_eta) {
-  float a = 0.0f, b = 0.0f;
+static inline void ref_deriv(int ng, const float* Dx, const float* Dy, const real* u, int i, int j, real* u_xi,
+                             real* u_eta) {
+  real a = (real)0, b = (real)0;
   for (int l = 0; l < ng; ++l) {
     a += Dx[i * ng + l] * u[j * ng + l];
     b += Dy[j * ng + l] * u[l * ng + i];
@@ -150,8 +192,8 @@ static inline void ref_deriv(int ng, const float* Dx, const float* Dy, const flo
 }
 
 /* kernels.cpp:53-62 and 181-190 */
-static inline float weak_div(int ng, const float* Dx, const float* Dy, const float* fx, const float* fe, int i, int j) {
-  float s = 0.0f;
+static inline real weak_div(int ng, const float* Dx, const float* Dy, const real* fx, const real* fe, int i, int j) {
+  real s = (real)0;
   for (int l = 0; l < ng; ++l) {
     s += Dx[l * ng + i] * fx[j * ng + l];
     s += Dy[l * ng + j] * fe[l * ng + i];
@@ -160,27 +202,27 @@ static inline float weak_div(int ng, const float* Dx, const float* Dy, const flo
 }
 
 /* pointwise.hpp:22-27 */
-static inline void phys_grad(float a_xi, float a_eta, float xi_x, float xi_z, float eta_x, float eta_z, float* a_x,
-                             float* a_z) {
+static inline void phys_grad(real a_xi, real a_eta, real xi_x, real xi_z, real eta_x, real eta_z, real* a_x,
+                             real* a_z) {
   *a_x = a_xi * xi_x + a_eta * eta_x;
   *a_z = a_xi * xi_z + a_eta * eta_z;
 }
 
 /* pointwise.hpp:33-38 */
-static inline void ref_flux(float wj, float qx, float qz, float xi_x, float xi_z, float eta_x, float eta_z, float* f_xi,
-                            float* f_eta) {
+static inline void ref_flux(real wj, real qx, real qz, real xi_x, real xi_z, real eta_x, real eta_z, real* f_xi,
+                            real* f_eta) {
   *f_xi = wj * (qx * xi_x + qz * xi_z);
   *f_eta = wj * (qx * eta_x + qz * eta_z);
 }
 
 /* kernels.cpp:18-38 */
-static inline void load_metric(int kind, const float* metric, int b, int npe, int k, float* xi_x, float* xi_z,
-                               float* eta_x, float* eta_z, float* jac) {
+static inline void load_metric(int kind, const float* metric, int b, int npe, int k, real* xi_x, real* xi_z,
+                               real* eta_x, real* eta_z, real* jac) {
   if (kind == INSEM_KIND_RECT) {
     const float* m = metric + (size_t)b * 3;
     *xi_x = m[0];
-    *xi_z = 0.0f;
-    *eta_x = 0.0f;
+    *xi_z = (real)0;
+    *eta_x = (real)0;
     *eta_z = m[1];
     *jac = m[2];
   } else {
@@ -197,26 +239,26 @@ static inline void load_metric(int kind, const float* metric, int b, int npe, in
 #define MAXNPE 81
 
 /* kernels.cpp:64-120 */
-static void elastic_cart(const insem_hip_setup* S, const insem_hip_block* B, const float* u, float* f) {
+static void elastic_cart(const insem_hip_setup* S, const insem_hip_block* B, const real* u, real* f) {
   const int ng = S->ngll, npe = ng * ng, kind = B->kind;
   const float *D = S->D, *w2 = S->w2;
 #pragma omp parallel for schedule(static)
   for (int b = 0; b < B->n_elements; ++b) {
     const size_t e = (size_t)B->elements[b];
-    const float* ux = u + e * 2 * npe;
-    const float* uz = ux + npe;
+    const real* ux = u + e * 2 * npe;
+    const real* uz = ux + npe;
     const float* lambda = B->coef0 + (size_t)b * npe;
     const float* mu = B->coef1 + (size_t)b * npe;
-    float f1x[MAXNPE], f1z[MAXNPE], f2x[MAXNPE], f2z[MAXNPE];
+    real f1x[MAXNPE], f1z[MAXNPE], f2x[MAXNPE], f2z[MAXNPE];
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) {
         const int k = j * ng + i;
-        float ux_xi, ux_eta, uz_xi, uz_eta;
+        real ux_xi, ux_eta, uz_xi, uz_eta;
         ref_deriv(ng, D, D, ux, i, j, &ux_xi, &ux_eta);
         ref_deriv(ng, D, D, uz, i, j, &uz_xi, &uz_eta);
-        float xi_x, xi_z, eta_x, eta_z, jac;
+        real xi_x, xi_z, eta_x, eta_z, jac;
         load_metric(kind, B->metric, b, npe, k, &xi_x, &xi_z, &eta_x, &eta_z, &jac);
-        float ux_x, ux_z, uz_x, uz_z;
+        real ux_x, ux_z, uz_x, uz_z;
         if (kind == INSEM_KIND_RECT) {
           ux_x = ux_xi * xi_x;
           ux_z = ux_eta * eta_z;
@@ -227,11 +269,11 @@ static void elastic_cart(const insem_hip_setup* S, const insem_hip_block* B, con
           phys_grad(uz_xi, uz_eta, xi_x, xi_z, eta_x, eta_z, &uz_x, &uz_z);
         }
         /* pointwise.hpp:12-19 */
-        const float div = ux_x + uz_z;
-        const float sxx = lambda[k] * div + 2.0f * mu[k] * ux_x;
-        const float szz = lambda[k] * div + 2.0f * mu[k] * uz_z;
-        const float sxz = mu[k] * (ux_z + uz_x);
-        const float wj = w2[k] * jac;
+        const real div = ux_x + uz_z;
+        const real sxx = lambda[k] * div + (real)2 * mu[k] * ux_x;
+        const real szz = lambda[k] * div + (real)2 * mu[k] * uz_z;
+        const real sxz = mu[k] * (ux_z + uz_x);
+        const real wj = w2[k] * jac;
         if (kind == INSEM_KIND_RECT) {
           f1x[k] = wj * sxx * xi_x;
           f1z[k] = wj * sxz * xi_x;
@@ -242,8 +284,8 @@ static void elastic_cart(const insem_hip_setup* S, const insem_hip_block* B, con
           ref_flux(wj, sxz, szz, xi_x, xi_z, eta_x, eta_z, &f1z[k], &f2z[k]);
         }
       }
-    float* fx = f + e * 2 * npe;
-    float* fz = fx + npe;
+    real* fx = f + e * 2 * npe;
+    real* fz = fx + npe;
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) {
         fx[j * ng + i] = -weak_div(ng, D, D, f1x, f2x, i, j);
@@ -253,50 +295,50 @@ static void elastic_cart(const insem_hip_setup* S, const insem_hip_block* B, con
 }
 
 /* kernels.cpp:122-158 */
-static void acoustic_cart(const insem_hip_setup* S, const insem_hip_block* B, const float* p, float* f) {
+static void acoustic_cart(const insem_hip_setup* S, const insem_hip_block* B, const real* p, real* f) {
   const int ng = S->ngll, npe = ng * ng, kind = B->kind;
   const float *D = S->D, *w2 = S->w2;
 #pragma omp parallel for schedule(static)
   for (int b = 0; b < B->n_elements; ++b) {
     const size_t e = (size_t)B->elements[b];
-    const float* pe = p + e * npe;
+    const real* pe = p + e * npe;
     const float* inv_rho = B->coef0 + (size_t)b * npe;
-    float f1[MAXNPE], f2[MAXNPE];
+    real f1[MAXNPE], f2[MAXNPE];
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) {
         const int k = j * ng + i;
-        float p_xi, p_eta;
+        real p_xi, p_eta;
         ref_deriv(ng, D, D, pe, i, j, &p_xi, &p_eta);
-        float xi_x, xi_z, eta_x, eta_z, jac;
+        real xi_x, xi_z, eta_x, eta_z, jac;
         load_metric(kind, B->metric, b, npe, k, &xi_x, &xi_z, &eta_x, &eta_z, &jac);
-        const float c = w2[k] * jac * inv_rho[k];
+        const real c = w2[k] * jac * inv_rho[k];
         if (kind == INSEM_KIND_RECT) {
           f1[k] = c * p_xi * xi_x * xi_x;
           f2[k] = c * p_eta * eta_z * eta_z;
         } else {
-          float p_x, p_z;
+          real p_x, p_z;
           phys_grad(p_xi, p_eta, xi_x, xi_z, eta_x, eta_z, &p_x, &p_z);
           ref_flux(c, p_x, p_z, xi_x, xi_z, eta_x, eta_z, &f1[k], &f2[k]);
         }
       }
-    float* fe = f + e * npe;
+    real* fe = f + e * npe;
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) fe[j * ng + i] = -weak_div(ng, D, D, f1, f2, i, j);
   }
 }
 
 /* kernels.cpp:195-204 */
-static void spread_axis(int ng, const float* Dx, const float* Dy, const float* m, const float* g, float* f) {
+static void spread_axis(int ng, const float* Dx, const float* Dy, const float* m, const real* g, real* f) {
   for (int jq = 0; jq < ng; ++jq) {
     const int kq = jq * ng;
-    const float xi_r = m[5 * kq], eta_r = m[5 * kq + 2];
+    const real xi_r = m[5 * kq], eta_r = m[5 * kq + 2];
     for (int ia = 0; ia < ng; ++ia) f[jq * ng + ia] -= g[jq] * xi_r * Dx[ia];
     for (int ja = 0; ja < ng; ++ja) f[ja * ng] -= g[jq] * eta_r * Dy[jq * ng + ja];
   }
 }
 
 /* kernels.cpp:206-263 */
-static void elastic_axi0(const insem_hip_setup* S, const insem_hip_block* B, const float* u, float* f) {
+static void elastic_axi0(const insem_hip_setup* S, const insem_hip_block* B, const real* u, real* f) {
   const int ng = S->ngll, npe = ng * ng;
 #pragma omp parallel for schedule(static)
   for (int b = 0; b < B->n_elements; ++b) {
@@ -304,43 +346,43 @@ static void elastic_axi0(const insem_hip_setup* S, const insem_hip_block* B, con
     const int axial = B->axial[b] != 0;
     const float* Dx = axial ? S->Dj : S->D;
     const float* Dy = S->D;
-    const float* ur = u + e * 2 * npe;
-    const float* uz = ur + npe;
+    const real* ur = u + e * 2 * npe;
+    const real* uz = ur + npe;
     const size_t p0 = (size_t)b * npe;
     const float* m = B->metric + p0 * 5;
-    float f1r[MAXNPE], f2r[MAXNPE], f1z[MAXNPE], f2z[MAXNPE], hoop_direct[MAXNPE], axis_g[MAXNG];
+    real f1r[MAXNPE], f2r[MAXNPE], f1z[MAXNPE], f2z[MAXNPE], hoop_direct[MAXNPE], axis_g[MAXNG];
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) {
         const int k = j * ng + i;
         const size_t p = p0 + k;
-        float ur_xi, ur_eta, uz_xi, uz_eta;
+        real ur_xi, ur_eta, uz_xi, uz_eta;
         ref_deriv(ng, Dx, Dy, ur, i, j, &ur_xi, &ur_eta);
         ref_deriv(ng, Dx, Dy, uz, i, j, &uz_xi, &uz_eta);
-        const float xi_r = m[5 * k], xi_z = m[5 * k + 1], eta_r = m[5 * k + 2], eta_z = m[5 * k + 3];
-        float ur_r, ur_z, uz_r, uz_z;
+        const real xi_r = m[5 * k], xi_z = m[5 * k + 1], eta_r = m[5 * k + 2], eta_z = m[5 * k + 3];
+        real ur_r, ur_z, uz_r, uz_z;
         phys_grad(ur_xi, ur_eta, xi_r, xi_z, eta_r, eta_z, &ur_r, &ur_z);
         phys_grad(uz_xi, uz_eta, xi_r, xi_z, eta_r, eta_z, &uz_r, &uz_z);
-        const float r = B->r[p];
-        const int on_axis = r == 0.0f;
-        const float e_tt = on_axis ? ur_r : ur[k] / r;
-        const float lambda = B->coef0[p], mu = B->coef1[p];
-        const float tr = ur_r + uz_z + e_tt;
-        const float srr = lambda * tr + 2 * mu * ur_r;
-        const float szz = lambda * tr + 2 * mu * uz_z;
-        const float stt = lambda * tr + 2 * mu * e_tt;
-        const float srz = mu * (ur_z + uz_r);
-        const float w = B->wr[p];
+        const real r = B->r[p];
+        const int on_axis = r == (real)0;
+        const real e_tt = on_axis ? ur_r : ur[k] / r;
+        const real lambda = B->coef0[p], mu = B->coef1[p];
+        const real tr = ur_r + uz_z + e_tt;
+        const real srr = lambda * tr + 2 * mu * ur_r;
+        const real szz = lambda * tr + 2 * mu * uz_z;
+        const real stt = lambda * tr + 2 * mu * e_tt;
+        const real srz = mu * (ur_z + uz_r);
+        const real w = B->wr[p];
         ref_flux(w, srr, srz, xi_r, xi_z, eta_r, eta_z, &f1r[k], &f2r[k]);
         ref_flux(w, srz, szz, xi_r, xi_z, eta_r, eta_z, &f1z[k], &f2z[k]);
         if (axial && i == 0) {
-          hoop_direct[k] = 0.0f;
+          hoop_direct[k] = (real)0;
           axis_g[j] = w * stt;
         } else {
           hoop_direct[k] = B->hoop[p] * stt;
         }
       }
-    float* fr = f + e * 2 * npe;
-    float* fz = fr + npe;
+    real* fr = f + e * 2 * npe;
+    real* fz = fr + npe;
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) {
         const int k = j * ng + i;
@@ -352,86 +394,86 @@ static void elastic_axi0(const insem_hip_setup* S, const insem_hip_block* B, con
 }
 
 /* kernels.cpp:265-294 */
-static void acoustic_axi0(const insem_hip_setup* S, const insem_hip_block* B, const float* pin, float* f) {
+static void acoustic_axi0(const insem_hip_setup* S, const insem_hip_block* B, const real* pin, real* f) {
   const int ng = S->ngll, npe = ng * ng;
 #pragma omp parallel for schedule(static)
   for (int b = 0; b < B->n_elements; ++b) {
     const size_t e = (size_t)B->elements[b];
     const float* Dx = B->axial[b] ? S->Dj : S->D;
     const float* Dy = S->D;
-    const float* pe = pin + e * npe;
+    const real* pe = pin + e * npe;
     const size_t p0 = (size_t)b * npe;
     const float* m = B->metric + p0 * 5;
-    float f1[MAXNPE], f2[MAXNPE];
+    real f1[MAXNPE], f2[MAXNPE];
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) {
         const int k = j * ng + i;
-        float p_xi, p_eta;
+        real p_xi, p_eta;
         ref_deriv(ng, Dx, Dy, pe, i, j, &p_xi, &p_eta);
-        const float xi_r = m[5 * k], xi_z = m[5 * k + 1], eta_r = m[5 * k + 2], eta_z = m[5 * k + 3];
-        float p_r, p_z;
+        const real xi_r = m[5 * k], xi_z = m[5 * k + 1], eta_r = m[5 * k + 2], eta_z = m[5 * k + 3];
+        real p_r, p_z;
         phys_grad(p_xi, p_eta, xi_r, xi_z, eta_r, eta_z, &p_r, &p_z);
         ref_flux(B->wr[p0 + k] * B->coef0[p0 + k], p_r, p_z, xi_r, xi_z, eta_r, eta_z, &f1[k], &f2[k]);
       }
-    float* fe = f + e * npe;
+    real* fe = f + e * npe;
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) fe[j * ng + i] = -weak_div(ng, Dx, Dy, f1, f2, i, j);
   }
 }
 
 /* kernels.cpp:305-385 */
-static void elastic_axi1(const insem_hip_setup* S, const insem_hip_block* B, const float* u, float* f) {
+static void elastic_axi1(const insem_hip_setup* S, const insem_hip_block* B, const real* u, real* f) {
   const int ng = S->ngll, npe = ng * ng;
-  const float s2 = (float)1.41421356237309504880;
-  const float is2 = (float)0.70710678118654752440;
+  const real s2 = (real)1.41421356237309504880;
+  const real is2 = (real)0.70710678118654752440;
 #pragma omp parallel for schedule(static)
   for (int b = 0; b < B->n_elements; ++b) {
     const size_t e = (size_t)B->elements[b];
     const int axial = B->axial[b] != 0;
     const float* Dx = axial ? S->Dj : S->D;
     const float* Dy = S->D;
-    const float* up = u + e * 3 * npe;
-    const float* um = up + npe;
-    const float* uz = um + npe;
+    const real* up = u + e * 3 * npe;
+    const real* um = up + npe;
+    const real* uz = um + npe;
     const size_t p0 = (size_t)b * npe;
     const float* m = B->metric + p0 * 5;
-    float f1p[MAXNPE], f2p[MAXNPE], f1m[MAXNPE], f2m[MAXNPE], f1z[MAXNPE], f2z[MAXNPE];
-    float direct_p[MAXNPE], direct_z[MAXNPE], axis_p[MAXNG], axis_z[MAXNG];
+    real f1p[MAXNPE], f2p[MAXNPE], f1m[MAXNPE], f2m[MAXNPE], f1z[MAXNPE], f2z[MAXNPE];
+    real direct_p[MAXNPE], direct_z[MAXNPE], axis_p[MAXNG], axis_z[MAXNG];
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) {
         const int k = j * ng + i;
         const size_t p = p0 + k;
-        float up_xi, up_eta, um_xi, um_eta, uz_xi, uz_eta;
+        real up_xi, up_eta, um_xi, um_eta, uz_xi, uz_eta;
         ref_deriv(ng, Dx, Dy, up, i, j, &up_xi, &up_eta);
         ref_deriv(ng, Dx, Dy, um, i, j, &um_xi, &um_eta);
         ref_deriv(ng, Dx, Dy, uz, i, j, &uz_xi, &uz_eta);
-        const float xi_r = m[5 * k], xi_z = m[5 * k + 1], eta_r = m[5 * k + 2], eta_z = m[5 * k + 3];
-        float up_r, up_z, um_r, um_z, uz_r, uz_z;
+        const real xi_r = m[5 * k], xi_z = m[5 * k + 1], eta_r = m[5 * k + 2], eta_z = m[5 * k + 3];
+        real up_r, up_z, um_r, um_z, uz_r, uz_z;
         phys_grad(up_xi, up_eta, xi_r, xi_z, eta_r, eta_z, &up_r, &up_z);
         phys_grad(um_xi, um_eta, xi_r, xi_z, eta_r, eta_z, &um_r, &um_z);
         phys_grad(uz_xi, uz_eta, xi_r, xi_z, eta_r, eta_z, &uz_r, &uz_z);
-        const float r = B->r[p];
-        const int on_axis = r == 0.0f;
-        const float up_over_r = on_axis ? up_r : up[k] / r;
-        const float uz_over_r = on_axis ? uz_r : uz[k] / r;
-        const float ur_r = (up_r + um_r) * is2, ur_z = (up_z + um_z) * is2;
-        const float ut_r = (up_r - um_r) * is2, ut_z = (up_z - um_z) * is2;
-        const float e_tt = s2 * up_over_r;
-        const float lambda = B->coef0[p], mu = B->coef1[p];
-        const float tr = ur_r + uz_z + e_tt;
-        const float srr = lambda * tr + 2 * mu * ur_r;
-        const float szz = lambda * tr + 2 * mu * uz_z;
-        const float stt = lambda * tr + 2 * mu * e_tt;
-        const float srz = mu * (ur_z + uz_r);
-        const float srt = mu * (ut_r - s2 * up_over_r);
-        const float stz = mu * (ut_z - uz_over_r);
-        const float w = B->wr[p];
+        const real r = B->r[p];
+        const int on_axis = r == (real)0;
+        const real up_over_r = on_axis ? up_r : up[k] / r;
+        const real uz_over_r = on_axis ? uz_r : uz[k] / r;
+        const real ur_r = (up_r + um_r) * is2, ur_z = (up_z + um_z) * is2;
+        const real ut_r = (up_r - um_r) * is2, ut_z = (up_z - um_z) * is2;
+        const real e_tt = s2 * up_over_r;
+        const real lambda = B->coef0[p], mu = B->coef1[p];
+        const real tr = ur_r + uz_z + e_tt;
+        const real srr = lambda * tr + 2 * mu * ur_r;
+        const real szz = lambda * tr + 2 * mu * uz_z;
+        const real stt = lambda * tr + 2 * mu * e_tt;
+        const real srz = mu * (ur_z + uz_r);
+        const real srt = mu * (ut_r - s2 * up_over_r);
+        const real stz = mu * (ut_z - uz_over_r);
+        const real w = B->wr[p];
         ref_flux(w, (srr + srt) * is2, (srz + stz) * is2, xi_r, xi_z, eta_r, eta_z, &f1p[k], &f2p[k]);
         ref_flux(w, (srr - srt) * is2, (srz - stz) * is2, xi_r, xi_z, eta_r, eta_z, &f1m[k], &f2m[k]);
         ref_flux(w, srz, szz, xi_r, xi_z, eta_r, eta_z, &f1z[k], &f2z[k]);
-        const float cp = s2 * (stt - srt), cz = -stz;
+        const real cp = s2 * (stt - srt), cz = -stz;
         if (axial && i == 0) {
-          direct_p[k] = direct_z[k] = 0.0f;
+          direct_p[k] = direct_z[k] = (real)0;
           axis_p[j] = w * cp;
           axis_z[j] = w * cz;
         } else {
@@ -439,9 +481,9 @@ static void elastic_axi1(const insem_hip_setup* S, const insem_hip_block* B, con
           direct_z[k] = B->hoop[p] * cz;
         }
       }
-    float* fp = f + e * 3 * npe;
-    float* fm = fp + npe;
-    float* fz = fm + npe;
+    real* fp = f + e * 3 * npe;
+    real* fm = fp + npe;
+    real* fz = fm + npe;
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) {
         const int k = j * ng + i;
@@ -457,7 +499,7 @@ static void elastic_axi1(const insem_hip_setup* S, const insem_hip_block* B, con
 }
 
 /* kernels.cpp:387-431 */
-static void acoustic_axi1(const insem_hip_setup* S, const insem_hip_block* B, const float* pin, float* f) {
+static void acoustic_axi1(const insem_hip_setup* S, const insem_hip_block* B, const real* pin, real* f) {
   const int ng = S->ngll, npe = ng * ng;
 #pragma omp parallel for schedule(static)
   for (int b = 0; b < B->n_elements; ++b) {
@@ -465,29 +507,29 @@ static void acoustic_axi1(const insem_hip_setup* S, const insem_hip_block* B, co
     const int axial = B->axial[b] != 0;
     const float* Dx = axial ? S->Dj : S->D;
     const float* Dy = S->D;
-    const float* pe = pin + e * npe;
+    const real* pe = pin + e * npe;
     const size_t p0 = (size_t)b * npe;
     const float* m = B->metric + p0 * 5;
-    float f1[MAXNPE], f2[MAXNPE], direct[MAXNPE], axis_g[MAXNG];
+    real f1[MAXNPE], f2[MAXNPE], direct[MAXNPE], axis_g[MAXNG];
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) {
         const int k = j * ng + i;
         const size_t p = p0 + k;
-        float p_xi, p_eta;
+        real p_xi, p_eta;
         ref_deriv(ng, Dx, Dy, pe, i, j, &p_xi, &p_eta);
-        const float xi_r = m[5 * k], xi_z = m[5 * k + 1], eta_r = m[5 * k + 2], eta_z = m[5 * k + 3];
-        float p_r, p_z;
+        const real xi_r = m[5 * k], xi_z = m[5 * k + 1], eta_r = m[5 * k + 2], eta_z = m[5 * k + 3];
+        real p_r, p_z;
         phys_grad(p_xi, p_eta, xi_r, xi_z, eta_r, eta_z, &p_r, &p_z);
-        const float inv_rho = B->coef0[p];
+        const real inv_rho = B->coef0[p];
         ref_flux(B->wr[p] * inv_rho, p_r, p_z, xi_r, xi_z, eta_r, eta_z, &f1[k], &f2[k]);
         if (axial && i == 0) {
-          direct[k] = 0.0f;
+          direct[k] = (real)0;
           axis_g[j] = B->wr[p] * inv_rho * p_r;
         } else {
           direct[k] = B->hoop[p] * inv_rho * pe[k] / B->r[p];
         }
       }
-    float* fe = f + e * npe;
+    real* fe = f + e * npe;
     for (int j = 0; j < ng; ++j)
       for (int i = 0; i < ng; ++i) {
         const int k = j * ng + i;
@@ -498,8 +540,8 @@ static void acoustic_axi1(const insem_hip_setup* S, const insem_hip_block* B, co
 }
 
 /* kernels.cpp:435-495: dispatch by physics, kind and azimuthal order */
-static int run_block(const insem_hip_setup* S, const insem_hip_domain* d, const insem_hip_block* B, const float* ul,
-                     float* fl) {
+static int run_block(const insem_hip_setup* S, const insem_hip_domain* d, const insem_hip_block* B, const real* ul,
+                     real* fl) {
   const int elastic = d->medium == INSEM_MEDIUM_ELASTIC;
   if (S->ngll < 2 || S->ngll > 9) return fail(INSEM_HIP_EINVAL, "no kernel for %d GLL points (supported: 2..9)", S->ngll);
   if (!(B->azimuthal_order == 0 || (B->azimuthal_order == 1 && B->kind == INSEM_KIND_AXISYM)))
@@ -524,14 +566,14 @@ static int run_block(const insem_hip_setup* S, const insem_hip_domain* d, const 
 }
 
 /* solver.cpp:82-87 */
-static double dot_fd(const float* a, const float* b, size_t n) {
+static double dot_fd(const real* a, const real* b, size_t n) {
   double s = 0.0;
   for (size_t i = 0; i < n; ++i) s += (double)a[i] * (double)b[i];
   return s;
 }
 
 /* solver.cpp:570-581 */
-static double internal_forces(oracle_ctx* c, int di, const float* u, int energy) {
+static double internal_forces(oracle_ctx* c, int di, const real* u, int energy) {
   const insem_hip_setup* S = c->s;
   const insem_hip_domain* d = &S->domains[di];
   dom_state* st = &c->st[di];
@@ -543,13 +585,13 @@ static double internal_forces(oracle_ctx* c, int di, const float* u, int energy)
 }
 
 /* solver.hpp:57-65 */
-static void apply_dirichlet(const insem_hip_domain* d, float* x) {
+static void apply_dirichlet(const insem_hip_domain* d, real* x) {
   const size_t ng = (size_t)d->n_global;
   for (int c = 0; c < d->ncomp; ++c)
-    for (int i = 0; i < d->n_dirichlet; ++i) x[c * ng + d->dirichlet_points[i]] = 0.0f;
+    for (int i = 0; i < d->n_dirichlet; ++i) x[c * ng + d->dirichlet_points[i]] = (real)0;
   for (int ci = 0; ci < d->n_axis_components; ++ci) {
     const int c = d->axis_components[ci];
-    for (int i = 0; i < d->n_axis; ++i) x[c * ng + d->axis_points[i]] = 0.0f;
+    for (int i = 0; i < d->n_axis; ++i) x[c * ng + d->axis_points[i]] = (real)0;
   }
 }
 
@@ -561,15 +603,15 @@ static void add_coupling(oracle_ctx* c, int di) {
   if (n == 0) return;
   const insem_hip_domain* d = &S->domains[di];
   const int pm = d->azimuthal_order == 1;
-  const float h = (float)KSQRT_HALF;
+  const real h = (real)KSQRT_HALF;
   if (d->medium == INSEM_MEDIUM_ELASTIC) {
     const int fi = dom_of(c, INSEM_MEDIUM_ACOUSTIC);
-    const float* pu = c->st[fi].u;
-    float* f = c->st[di].f;
+    const real* pu = c->st[fi].u;
+    real* f = c->st[di].f;
     const size_t ng = (size_t)d->n_global;
     const size_t zc = pm ? 2 * ng : ng;
     for (int i = 0; i < n; ++i) {
-      const float p = pu[cp->fluid_points[i]];
+      const real p = pu[cp->fluid_points[i]];
       const size_t g = (size_t)cp->solid_points[i];
       if (pm) {
         f[g] += h * cp->cx[i] * p;
@@ -581,13 +623,13 @@ static void add_coupling(oracle_ctx* c, int di) {
     }
   } else {
     const int si = dom_of(c, INSEM_MEDIUM_ELASTIC);
-    const float* a = c->st[si].a;
+    const real* a = c->st[si].a;
     const size_t ngs = (size_t)S->domains[si].n_global;
     const size_t zc = pm ? 2 * ngs : ngs;
-    float* f = c->st[di].f;
+    real* f = c->st[di].f;
     for (int i = 0; i < n; ++i) {
       const size_t gs = (size_t)cp->solid_points[i];
-      const float ar = pm ? h * (a[gs] + a[ngs + gs]) : a[gs];
+      const real ar = pm ? h * (a[gs] + a[ngs + gs]) : a[gs];
       f[cp->fluid_points[i]] -= cp->cx[i] * ar + cp->cz[i] * a[zc + gs];
     }
   }
@@ -631,7 +673,7 @@ static void solve_acceleration(oracle_ctx* c, int di, int damping_implicit) {
       }
     }
     solve_spd(nc, A, rhs);
-    for (int r = 0; r < nc; ++r) st->a[r * ng + g] = (float)rhs[r];
+    for (int r = 0; r < nc; ++r) st->a[r * ng + g] = (real)rhs[r];
   }
   apply_dirichlet(d, st->a);
 }
@@ -650,7 +692,7 @@ static void compute_acceleration(oracle_ctx* c, int energy, int damping_implicit
     for (int s = 0; s < S->n_sources; ++s) {
       const insem_hip_source* src = &S->sources[s];
       if (src->domain != di) continue;
-      const float amp = c->amp[(size_t)s * (c->n_steps + 1) + c->step];
+      const real amp = c->amp[(size_t)s * (c->n_steps + 1) + c->step];
       const size_t ng = (size_t)d->n_global, np = (size_t)src->n_points;
       for (int cc = 0; cc < d->ncomp; ++cc)
         for (size_t i = 0; i < np; ++i) st->f[cc * ng + src->points[i]] += src->coef[cc * np + i] * amp;
@@ -689,7 +731,7 @@ static int grow_samples(oracle_ctx* c) {
   return 0;
 }
 
-static double interp(const insem_hip_receiver* r, const float* field, size_t ng, int comp) {
+static double interp(const insem_hip_receiver* r, const real* field, size_t ng, int comp) {
   double s = 0.0;
   for (int i = 0; i < r->n_points; ++i) s += (double)r->value[i] * field[comp * ng + r->points[i]];
   return s;
@@ -714,7 +756,7 @@ static void record(oracle_ctx* c) {
         out[1] = c->pv[ri].vel_z;
         continue;
       }
-      const float* field = q == INSEM_Q_VELOCITY ? st->v : q == INSEM_Q_ACCELERATION ? st->a : st->u;
+      const real* field = q == INSEM_Q_VELOCITY ? st->v : q == INSEM_Q_ACCELERATION ? st->a : st->u;
       if (d->medium == INSEM_MEDIUM_ELASTIC && d->azimuthal_order == 1) {
         const double up = interp(r, field, ng, 0), um = interp(r, field, ng, 1);
         out[0] = KSQRT_HALF * (up + um);
@@ -744,7 +786,7 @@ static void push_energy(oracle_ctx* c, double t, double kin, double pot) {
 
 static void pressure_gradient(const oracle_ctx* c, int ri, double* gx, double* gz) {
   const insem_hip_receiver* r = &c->s->receivers[ri];
-  const float* u = c->st[r->domain].u;
+  const real* u = c->st[r->domain].u;
   double x = 0.0, z = 0.0;
   for (int i = 0; i < r->n_points; ++i) {
     x += (double)r->d_dx[i] * u[r->points[i]];
@@ -783,12 +825,12 @@ int oracle_create(const insem_hip_setup* s, oracle_ctx** out) {
   for (int di = 0; di < s->n_domains; ++di) {
     const insem_hip_domain* d = &s->domains[di];
     const size_t n = dom_len(d), nl = (size_t)d->ncomp * d->n_elements * npe;
-    c->st[di].u = (float*)calloc(n, sizeof(float));
-    c->st[di].v = (float*)calloc(n, sizeof(float));
-    c->st[di].a = (float*)calloc(n, sizeof(float));
-    c->st[di].f = (float*)calloc(n, sizeof(float));
-    c->st[di].ul = (float*)calloc(nl, sizeof(float));
-    c->st[di].fl = (float*)calloc(nl, sizeof(float));
+    c->st[di].u = (real*)calloc(n, sizeof(real));
+    c->st[di].v = (real*)calloc(n, sizeof(real));
+    c->st[di].a = (real*)calloc(n, sizeof(real));
+    c->st[di].f = (real*)calloc(n, sizeof(real));
+    c->st[di].ul = (real*)calloc(nl, sizeof(real));
+    c->st[di].fl = (real*)calloc(nl, sizeof(real));
   }
   int nt = 0;
   for (int r = 0; r < s->n_receivers; ++r) nt += s->receivers[r].n_quantities;
@@ -839,8 +881,8 @@ int oracle_set_time(oracle_ctx* c, double dt, int32_t n_steps, const float* ampl
   c->n_steps = n_steps;
   free(c->amp_copy);
   const size_t n = (size_t)c->s->n_sources * ((size_t)n_steps + 1);
-  c->amp_copy = (float*)calloc(n + 1, sizeof(float));
-  if (n && amplitudes) memcpy(c->amp_copy, amplitudes, n * sizeof(float));
+  c->amp_copy = (real*)calloc(n + 1, sizeof(real));
+  if (n && amplitudes) from_float(c->amp_copy, amplitudes, n);
   c->amp = c->amp_copy;
   c->have_time = 1;
   return INSEM_HIP_OK;
@@ -856,8 +898,8 @@ int oracle_lambda_max(oracle_ctx* c, const float* start, double* lambda_out) {
     const insem_hip_domain* d = &S->domains[di];
     dom_state* st = &c->st[di];
     const size_t ng = (size_t)d->n_global, len = dom_len(d);
-    float* x = (float*)malloc(len * sizeof(float));
-    memcpy(x, start + off, len * sizeof(float));
+    real* x = (real*)malloc(len * sizeof(real));
+    from_float(x, start + off, len);
     off += len;
     for (int it = 0; it < 100; ++it) {
       apply_dirichlet(d, x);
@@ -869,32 +911,32 @@ int oracle_lambda_max(oracle_ctx* c, const float* start, double* lambda_out) {
       for (int cc = 0; cc < d->ncomp; ++cc)
         for (size_t g = 0; g < ng; ++g) {
           const double y = -(double)st->f[cc * ng + g] * d->inv_mass[g];
-          x[cc * ng + g] = (float)y;
+          x[cc * ng + g] = (real)y;
           norm += y * y;
         }
       if (norm == 0.0) break;
       const double scale = 1.0 / sqrt(norm);
-      for (size_t i = 0; i < len; ++i) x[i] = (float)(x[i] * scale);
+      for (size_t i = 0; i < len; ++i) x[i] = (real)(x[i] * scale);
     }
-    memset(st->f, 0, len * sizeof(float));
+    memset(st->f, 0, len * sizeof(real));
     free(x);
   }
   if (S->coupling.n_points > 0) {
     const int si = dom_of(c, INSEM_MEDIUM_ELASTIC), fi = dom_of(c, INSEM_MEDIUM_ACOUSTIC);
     const int order[2] = {si, fi};
-    float* saved[4];
+    real* saved[4];
     const size_t len_s = dom_len(&S->domains[si]), len_f = dom_len(&S->domains[fi]);
-    saved[0] = (float*)malloc(len_s * sizeof(float));
-    saved[1] = (float*)malloc(len_f * sizeof(float));
-    saved[2] = (float*)malloc(len_s * sizeof(float));
-    saved[3] = (float*)malloc(len_f * sizeof(float));
-    memcpy(saved[0], c->st[si].u, len_s * sizeof(float));
-    memcpy(saved[1], c->st[fi].u, len_f * sizeof(float));
-    memcpy(saved[2], c->st[si].a, len_s * sizeof(float));
-    memcpy(saved[3], c->st[fi].a, len_f * sizeof(float));
-    memcpy(c->st[si].u, start + off, len_s * sizeof(float));
+    saved[0] = (real*)malloc(len_s * sizeof(real));
+    saved[1] = (real*)malloc(len_f * sizeof(real));
+    saved[2] = (real*)malloc(len_s * sizeof(real));
+    saved[3] = (real*)malloc(len_f * sizeof(real));
+    memcpy(saved[0], c->st[si].u, len_s * sizeof(real));
+    memcpy(saved[1], c->st[fi].u, len_f * sizeof(real));
+    memcpy(saved[2], c->st[si].a, len_s * sizeof(real));
+    memcpy(saved[3], c->st[fi].a, len_f * sizeof(real));
+    from_float(c->st[si].u, start + off, len_s);
     off += len_s;
-    memcpy(c->st[fi].u, start + off, len_f * sizeof(float));
+    from_float(c->st[fi].u, start + off, len_f);
     double lam = 0.0;
     for (int it = 0; it < 120; ++it) {
       for (int o = 0; o < 2; ++o) apply_dirichlet(&S->domains[order[o]], c->st[order[o]].u);
@@ -934,15 +976,15 @@ int oracle_lambda_max(oracle_ctx* c, const float* start, double* lambda_out) {
       for (int o = 0; o < 2; ++o) {
         dom_state* st = &c->st[order[o]];
         const size_t len = dom_len(&S->domains[order[o]]);
-        for (size_t i = 0; i < len; ++i) st->u[i] = (float)(st->a[i] * scale);
+        for (size_t i = 0; i < len; ++i) st->u[i] = (real)(st->a[i] * scale);
       }
     }
-    memcpy(c->st[si].u, saved[0], len_s * sizeof(float));
-    memcpy(c->st[fi].u, saved[1], len_f * sizeof(float));
-    memcpy(c->st[si].a, saved[2], len_s * sizeof(float));
-    memcpy(c->st[fi].a, saved[3], len_f * sizeof(float));
-    memset(c->st[si].f, 0, len_s * sizeof(float));
-    memset(c->st[fi].f, 0, len_f * sizeof(float));
+    memcpy(c->st[si].u, saved[0], len_s * sizeof(real));
+    memcpy(c->st[fi].u, saved[1], len_f * sizeof(real));
+    memcpy(c->st[si].a, saved[2], len_s * sizeof(real));
+    memcpy(c->st[fi].a, saved[3], len_f * sizeof(real));
+    memset(c->st[si].f, 0, len_s * sizeof(real));
+    memset(c->st[fi].f, 0, len_f * sizeof(real));
     for (int k = 0; k < 4; ++k) free(saved[k]);
     if (lam > lambda) lambda = lam;
   }
@@ -956,8 +998,8 @@ int oracle_set_state(oracle_ctx* c, int32_t di, const float* u, const float* v) 
   if (c->initialized) return fail(INSEM_HIP_ESTATE, "initial conditions must be set before the first time step");
   const insem_hip_domain* d = &c->s->domains[di];
   const size_t n = dom_len(d);
-  memcpy(c->st[di].u, u, n * sizeof(float));
-  memcpy(c->st[di].v, v, n * sizeof(float));
+  from_float(c->st[di].u, u, n);
+  from_float(c->st[di].v, v, n);
   apply_dirichlet(d, c->st[di].u);
   apply_dirichlet(d, c->st[di].v);
   return INSEM_HIP_OK;
@@ -966,8 +1008,8 @@ int oracle_set_state(oracle_ctx* c, int32_t di, const float* u, const float* v) 
 int oracle_get_field(oracle_ctx* c, int32_t di, int32_t field, float* out) {
   if (!c || di < 0 || di >= c->s->n_domains) return fail(INSEM_HIP_EINVAL, "invalid domain");
   const dom_state* st = &c->st[di];
-  const float* src = field == INSEM_FIELD_U ? st->u : field == INSEM_FIELD_V ? st->v : field == INSEM_FIELD_A ? st->a : st->f;
-  memcpy(out, src, dom_len(&c->s->domains[di]) * sizeof(float));
+  const real* src = field == INSEM_FIELD_U ? st->u : field == INSEM_FIELD_V ? st->v : field == INSEM_FIELD_A ? st->a : st->f;
+  to_float(out, src, dom_len(&c->s->domains[di]));
   return INSEM_HIP_OK;
 }
 
@@ -978,14 +1020,14 @@ int oracle_advance(oracle_ctx* c, int32_t n) {
   if (!c->initialized) initialize(c);
   const insem_hip_setup* S = c->s;
   const double dt = c->dt;
-  const float rdt = (float)dt;
-  const float half_dt = (float)(0.5 * dt);
-  const float half_dt2 = (float)(0.5 * dt * dt);
+  const real rdt = (real)dt;
+  const real half_dt = (real)(0.5 * dt);
+  const real half_dt2 = (real)(0.5 * dt * dt);
   for (int s = 0; s < n && c->step < c->n_steps; ++s) {
     for (int di = 0; di < S->n_domains; ++di) {
-      float* u = c->st[di].u;
-      float* v = c->st[di].v;
-      const float* a = c->st[di].a;
+      real* u = c->st[di].u;
+      real* v = c->st[di].v;
+      const real* a = c->st[di].a;
       const long len = (long)dom_len(&S->domains[di]);
 #pragma omp parallel for schedule(static)
       for (long i = 0; i < len; ++i) {
@@ -997,8 +1039,8 @@ int oracle_advance(oracle_ctx* c, int32_t n) {
     const int energy = S->energy_every > 0 && c->step % S->energy_every == 0;
     compute_acceleration(c, energy, 1);
     for (int di = 0; di < S->n_domains; ++di) {
-      float* v = c->st[di].v;
-      const float* a = c->st[di].a;
+      real* v = c->st[di].v;
+      const real* a = c->st[di].a;
       const long len = (long)dom_len(&S->domains[di]);
 #pragma omp parallel for schedule(static)
       for (long i = 0; i < len; ++i) v[i] += half_dt * a[i];
@@ -1057,8 +1099,12 @@ int oracle_read_energy(oracle_ctx* c, double* out) {
 
 int oracle_internal_forces(oracle_ctx* c, int32_t di, const float* u, float* f, double* dot) {
   if (!c || di < 0 || di >= c->s->n_domains) return fail(INSEM_HIP_EINVAL, "invalid domain");
-  const double d = internal_forces(c, di, u, dot != NULL);
-  memcpy(f, c->st[di].f, dom_len(&c->s->domains[di]) * sizeof(float));
+  const size_t n = dom_len(&c->s->domains[di]);
+  real* x = (real*)calloc(n + 1, sizeof(real));
+  from_float(x, u, n);
+  const double d = internal_forces(c, di, x, dot != NULL);
+  free(x);
+  to_float(f, c->st[di].f, n);
   if (dot) *dot = d;
   return INSEM_HIP_OK;
 }
@@ -1067,17 +1113,37 @@ int oracle_block_forces(oracle_ctx* c, int32_t di, int32_t bi, const float* ul, 
   if (!c || di < 0 || di >= c->s->n_domains) return fail(INSEM_HIP_EINVAL, "invalid domain");
   const insem_hip_domain* d = &c->s->domains[di];
   if (bi < 0 || bi >= d->n_blocks) return fail(INSEM_HIP_EINVAL, "invalid block");
-  return run_block(c->s, d, &d->blocks[bi], ul, fl);
+  const size_t n = (size_t)d->n_elements * d->ncomp * c->s->ngll * c->s->ngll;
+  dom_state* st = &c->st[di];
+  from_float(st->ul, ul, n);
+  from_float(st->fl, fl, n);
+  const int rc = run_block(c->s, d, &d->blocks[bi], st->ul, st->fl);
+  to_float(fl, st->fl, n);
+  return rc;
 }
 
 int oracle_gather(oracle_ctx* c, int32_t di, const float* g, float* l) {
   if (!c || di < 0 || di >= c->s->n_domains) return fail(INSEM_HIP_EINVAL, "invalid domain");
-  gather(&c->s->domains[di], c->s->ngll * c->s->ngll, g, l);
+  const insem_hip_domain* d = &c->s->domains[di];
+  const size_t n = dom_len(d), nl = (size_t)d->n_elements * d->ncomp * c->s->ngll * c->s->ngll;
+  dom_state* st = &c->st[di];
+  real* x = (real*)calloc(n + 1, sizeof(real));
+  from_float(x, g, n);
+  gather(d, c->s->ngll * c->s->ngll, x, st->ul);
+  free(x);
+  to_float(l, st->ul, nl);
   return INSEM_HIP_OK;
 }
 
 int oracle_assemble(oracle_ctx* c, int32_t di, const float* l, float* g) {
   if (!c || di < 0 || di >= c->s->n_domains) return fail(INSEM_HIP_EINVAL, "invalid domain");
-  assemble(&c->s->domains[di], c->s->ngll * c->s->ngll, l, g);
+  const insem_hip_domain* d = &c->s->domains[di];
+  const size_t n = dom_len(d), nl = (size_t)d->n_elements * d->ncomp * c->s->ngll * c->s->ngll;
+  dom_state* st = &c->st[di];
+  real* x = (real*)calloc(n + 1, sizeof(real));
+  from_float(st->fl, l, nl);
+  assemble(d, c->s->ngll * c->s->ngll, st->fl, x);
+  to_float(g, x, n);
+  free(x);
   return INSEM_HIP_OK;
 }

@@ -19,18 +19,20 @@ if PKG not in sys.path:
     sys.path.insert(0, PKG)
 
 ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+# the same restatement computing in double (the reference's INSEM_REAL=double build)
+ORACLE64_PATH = os.path.join(ROOT, "oracle", "liboracle64.so")
 
-_lib = None
+_libs = {}
 P = ctypes.POINTER
 F32 = P(ctypes.c_float)
 F64 = P(ctypes.c_double)
 I32 = P(ctypes.c_int32)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        L = ctypes.CDLL(ORACLE_PATH)
+def lib(double=False):
+    path = ORACLE64_PATH if double else ORACLE_PATH
+    if path not in _libs:
+        L = ctypes.CDLL(path)
         sig = {
             "oracle_last_error": (ctypes.c_char_p, []),
             "oracle_create": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_void_p)]),
@@ -55,13 +57,13 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
-def _ok(rc):
+def _ok(rc, L=None):
     if rc != 0:
-        raise RuntimeError(lib().oracle_last_error().decode())
+        raise RuntimeError((L or lib()).oracle_last_error().decode())
 
 
 def f32(a):
@@ -73,14 +75,16 @@ def f64(a):
 
 
 class OracleRun:
-    """The oracle's time loop on one setup image (the image must outlive it)."""
+    """The oracle's time loop on one setup image (the image must outlive it);
+    double=True runs the double-arithmetic build."""
 
-    def __init__(self, image, dt=None, lambda_max=None, threads=0):
+    def __init__(self, image, dt=None, lambda_max=None, threads=0, double=False):
         self.image = image
+        self.L = L = lib(double)
         if threads:
-            lib().oracle_set_threads(threads)
+            L.oracle_set_threads(threads)
         h = ctypes.c_void_p()
-        _ok(lib().oracle_create(ctypes.c_void_p(image.view), ctypes.byref(h)))
+        _ok(L.oracle_create(ctypes.c_void_p(image.view), ctypes.byref(h)), L)
         self.h = h
         t = image.time
         if dt is None:
@@ -96,40 +100,40 @@ class OracleRun:
         else:
             self.n_steps = 0
         amps = image.amplitudes(self.dt, self.n_steps)
-        _ok(lib().oracle_set_time(self.h, self.dt, self.n_steps, f32(amps)))
+        _ok(self.L.oracle_set_time(self.h, self.dt, self.n_steps, f32(amps)), self.L)
 
     def lambda_max(self):
         start = self.image.start_vectors()
         out = ctypes.c_double()
-        _ok(lib().oracle_lambda_max(self.h, f32(start), ctypes.byref(out)))
+        _ok(self.L.oracle_lambda_max(self.h, f32(start), ctypes.byref(out)), self.L)
         return out.value
 
     def set_state(self, di, u, v):
         u = np.ascontiguousarray(u, dtype=np.float32)
         v = np.ascontiguousarray(v, dtype=np.float32)
-        _ok(lib().oracle_set_state(self.h, di, f32(u), f32(v)))
+        _ok(self.L.oracle_set_state(self.h, di, f32(u), f32(v)), self.L)
 
     def field(self, di, which):
         d = self.image.domains[di]
         out = np.zeros((d["ncomp"], d["n_global"]), dtype=np.float32)
-        _ok(lib().oracle_get_field(self.h, di, which, f32(out)))
+        _ok(self.L.oracle_get_field(self.h, di, which, f32(out)), self.L)
         return out
 
     def advance(self, n):
-        _ok(lib().oracle_advance(self.h, int(n)))
+        _ok(self.L.oracle_advance(self.h, int(n)), self.L)
 
     def run(self):
         s = ctypes.c_int32()
-        _ok(lib().oracle_step(self.h, ctypes.byref(s)))
+        _ok(self.L.oracle_step(self.h, ctypes.byref(s)), self.L)
         self.advance(self.n_steps - s.value)
 
     def traces(self):
         ns = ctypes.c_int32()
-        _ok(lib().oracle_n_samples(self.h, ctypes.byref(ns)))
+        _ok(self.L.oracle_n_samples(self.h, ctypes.byref(ns)), self.L)
         n = ns.value
         total = sum(nc for _, _, nc in self.image.traces) * n
         buf = np.zeros(max(total, 1))
-        _ok(lib().oracle_read_traces(self.h, f64(buf)))
+        _ok(self.L.oracle_read_traces(self.h, f64(buf)), self.L)
         out, o = {}, 0
         for name, q, nc in self.image.traces:
             out[(name, q)] = buf[o:o + nc * n].reshape(n, nc).copy()
@@ -138,9 +142,9 @@ class OracleRun:
 
     def energy(self):
         ne = ctypes.c_int32()
-        _ok(lib().oracle_n_energy(self.h, ctypes.byref(ne)))
+        _ok(self.L.oracle_n_energy(self.h, ctypes.byref(ne)), self.L)
         buf = np.zeros((max(ne.value, 1), 3))
-        _ok(lib().oracle_read_energy(self.h, f64(buf)))
+        _ok(self.L.oracle_read_energy(self.h, f64(buf)), self.L)
         return buf[:ne.value]
 
     def internal_forces(self, di, u, with_dot=True):
@@ -148,32 +152,32 @@ class OracleRun:
         u = np.ascontiguousarray(u, dtype=np.float32)
         f = np.zeros((d["ncomp"], d["n_global"]), dtype=np.float32)
         dot = ctypes.c_double()
-        _ok(lib().oracle_internal_forces(self.h, di, f32(u), f32(f), ctypes.byref(dot) if with_dot else None))
+        _ok(self.L.oracle_internal_forces(self.h, di, f32(u), f32(f), ctypes.byref(dot) if with_dot else None), self.L)
         return f, dot.value
 
     def block_forces(self, di, bi, ul, fl):
         ul = np.ascontiguousarray(ul, dtype=np.float32)
         fl = np.ascontiguousarray(fl, dtype=np.float32).copy()
-        _ok(lib().oracle_block_forces(self.h, di, bi, f32(ul), f32(fl)))
+        _ok(self.L.oracle_block_forces(self.h, di, bi, f32(ul), f32(fl)), self.L)
         return fl
 
     def gather(self, di, g):
         d = self.image.domains[di]
         g = np.ascontiguousarray(g, dtype=np.float32)
         out = np.zeros(d["n_elements"] * d["ncomp"] * self.image.ngll ** 2, dtype=np.float32)
-        _ok(lib().oracle_gather(self.h, di, f32(g), f32(out)))
+        _ok(self.L.oracle_gather(self.h, di, f32(g), f32(out)), self.L)
         return out
 
     def assemble(self, di, loc):
         d = self.image.domains[di]
         loc = np.ascontiguousarray(loc, dtype=np.float32)
         out = np.zeros((d["ncomp"], d["n_global"]), dtype=np.float32)
-        _ok(lib().oracle_assemble(self.h, di, f32(loc), f32(out)))
+        _ok(self.L.oracle_assemble(self.h, di, f32(loc), f32(out)), self.L)
         return out
 
     def close(self):
         if getattr(self, "h", None):
-            lib().oracle_destroy(self.h)
+            self.L.oracle_destroy(self.h)
             self.h = None
 
     def __del__(self):
