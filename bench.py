@@ -268,23 +268,9 @@ def served_leg(n_requests, rate, sweep, slots=4, batch=8, seed=7):
     import numpy as np
 
     from insem_amd import serve
-    from insem_amd.configs import cased_hole, layered_coupled, open_hole
+    from insem_amd.configs import served_request as build, served_stream
 
-    rng = np.random.default_rng(seed)
-    unit_arrivals = np.cumsum(rng.exponential(1.0, n_requests))
-    kinds = rng.choice(["cased", "open", "layered"], p=[0.6, 0.3, 0.1], size=n_requests)
-    seeds = rng.integers(0, 2**31, size=n_requests)
-
-    def build(kind, s):
-        r = np.random.default_rng(int(s))
-        if kind == "cased":
-            return cased_hole(r, duration=r.uniform(3e-3, 8e-3))
-        if kind == "open":
-            return open_hole(r, duration=r.uniform(3e-3, 8e-3))
-        nx = int(r.integers(250, 501)) * 2  # 1-4 M nodes at order 4
-        nzf, nzs = nx // 10, (nx * 2) // 10
-        return layered_coupled(nx=nx, nz_fluid=nzf, nz_solid=(nzs, nzs), duration=r.uniform(0.5, 1.0),
-                               n_receivers=101)
+    unit_arrivals, kinds, seeds = served_stream(n_requests, seed)
 
     stc = dict(slowness=np.linspace(100e-6, 1500e-6, 141), window=5e-4, times=np.linspace(0.0, 2.5e-3, 51))
 

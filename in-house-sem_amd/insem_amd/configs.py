@@ -192,3 +192,29 @@ def open_hole(rng, duration=3e-3, z=(-1.5, 4.5), formation_radius=4.0, dz=0.08, 
         sim.add_receiver(f"r{k}", 0.05, receiver_z0 + 0.15 * k, ["pressure"])
     sim.set_time(duration=duration)
     return sim
+
+
+def served_request(kind, seed):
+    """One request of the served workload: "cased" (cased-hole dipole, 3-8 ms),
+    "open" (open-hole monopole, 3-8 ms) or "layered" (2D coupled layered
+    model of 1-4 M nodes at order 4, 0.5-1 s, 101 receivers), drawn from
+    numpy.random.default_rng(seed)."""
+    r = np.random.default_rng(int(seed))
+    if kind == "cased":
+        return cased_hole(r, duration=r.uniform(3e-3, 8e-3))
+    if kind == "open":
+        return open_hole(r, duration=r.uniform(3e-3, 8e-3))
+    nx = int(r.integers(250, 501)) * 2
+    nzf, nzs = nx // 10, (nx * 2) // 10
+    return layered_coupled(nx=nx, nz_fluid=nzf, nz_solid=(nzs, nzs), duration=r.uniform(0.5, 1.0), n_receivers=101)
+
+
+def served_stream(n_requests, seed=7):
+    """The served workload's Poisson stream at unit rate: (arrival times,
+    kinds, seeds), 60 % cased-hole, 30 % open-hole and 10 % layered
+    requests; scale the arrival times by 1 / rate for another rate."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0, n_requests))
+    kinds = rng.choice(["cased", "open", "layered"], p=[0.6, 0.3, 0.1], size=n_requests)
+    seeds = rng.integers(0, 2**31, size=n_requests)
+    return arrivals, kinds, seeds
