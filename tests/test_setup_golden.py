@@ -47,7 +47,7 @@ def test_quadrature_rules_bitwise():
 def test_fixtures_cover_hanging_faces_and_every_kind():
     models = GOLDEN["models"]
     # constrained maps (hanging faces) in the refined and borehole models
-    for name in ("borehole_m0", "borehole_m1", "borehole_m1_order6", "refined_sheared_grid"):
+    for name in ("borehole_m0", "borehole_m1", "refined_sheared_grid"):
         assert any(k.endswith("row_dofs") and v["shape"][0] > 0 for k, v in models[name].items()), name
     kinds = {int(v["head"][0]) for m in models.values() for k, v in m.items() if k.endswith(".info") and ".b" in k}
     assert kinds == {0, 1, 2, 3}
