@@ -31,11 +31,19 @@ from insem_amd import configs, mesh  # noqa: E402
 from setup_arrays import setup_arrays  # noqa: E402
 
 OUT = os.path.join(HERE, "setup_digests.json")
+# full copies of the float64 arrays (coordinates, constraint weights, rules):
+# compared within a few ulps, every other array by its digest
+OUT64 = os.path.join(HERE, "setup_float64.npz")
 FIXED_DT = 1.234e-6  # source amplitude tables are digested at this dt, 40 steps
 
 
-def digest(a):
+FULL64 = {}  # name -> float64 array, filled by digest(..., name=...)
+
+
+def digest(a, name=None):
     a = np.ascontiguousarray(a)
+    if name is not None and a.dtype == np.float64:
+        FULL64[name] = a.copy()
     head = a.ravel()[:6]
     return {"sha256": hashlib.sha256(a.tobytes()).hexdigest(), "shape": list(a.shape), "dtype": str(a.dtype),
             "head": [float(x) for x in head]}
@@ -112,11 +120,11 @@ MODELS = {
 }
 
 
-def model_digests(sim):
+def model_digests(sim, name=""):
     img = sim.build_setup()
-    out = {k: digest(v) for k, v in setup_arrays(img.view).items()}
+    out = {k: digest(v, f"{name}/{k}") for k, v in setup_arrays(img.view).items()}
     for di in range(img.n_domains):
-        out[f"d{di}.coords"] = digest(img.coordinates(di))
+        out[f"d{di}.coords"] = digest(img.coordinates(di), f"{name}/d{di}.coords")
     out["start_vectors"] = digest(img.start_vectors())
     out["amplitudes"] = digest(img.amplitudes(FIXED_DT, 40))
     out["traces"] = {"names": [list(t) for t in img.traces]}
@@ -139,7 +147,7 @@ def mesh_digests():
     out = {}
     for name, m in cases.items():
         for field in ("nodes", "cells", "material", "boundary"):
-            out[f"{name}.{field}"] = digest(np.asarray(getattr(m, field)))
+            out[f"{name}.{field}"] = digest(np.asarray(getattr(m, field)), f"meshes/{name}.{field}")
         out[f"{name}.kind"] = {"cell_type": m.cell_type, "coords": m.coords}
     return out
 
@@ -149,13 +157,13 @@ def basis_digests():
     for n in range(2, 11):
         for a, b in ((0, 0), (0, 1)):
             x, w = insem.gauss_lobatto_jacobi(n, a, b)
-            out[f"rule{n}.{a}{b}.x"] = digest(x)
-            out[f"rule{n}.{a}{b}.w"] = digest(w)
+            out[f"rule{n}.{a}{b}.x"] = digest(x, f"basis/rule{n}.{a}{b}.x")
+            out[f"rule{n}.{a}{b}.w"] = digest(w, f"basis/rule{n}.{a}{b}.w")
     return out
 
 
 def compute():
-    return {"models": {k: model_digests(f()) for k, f in MODELS.items()}, "meshes": mesh_digests(),
+    return {"models": {k: model_digests(f(), k) for k, f in MODELS.items()}, "meshes": mesh_digests(),
             "basis": basis_digests()}
 
 
@@ -163,5 +171,6 @@ if __name__ == "__main__":
     data = compute()
     with open(OUT, "w") as fh:
         json.dump(data, fh, indent=0, sort_keys=True)
+    np.savez_compressed(OUT64, **FULL64)
     print(f"wrote {OUT}: {sum(len(v) for v in data['models'].values())} model arrays, "
           f"{len(data['meshes'])} mesh arrays, {len(data['basis'])} basis arrays")
