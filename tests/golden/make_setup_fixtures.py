@@ -1,5 +1,6 @@
-"""Writes tests/golden/setup_digests.json: SHA-256 digests (with shape, dtype
-and the first values) of every array the host setup produces for a set of
+"""Writes tests/golden/setup_digests.json: SHA-256 digests (with shape, dtype,
+the first values and, for floating arrays, a position-sensitive
+fingerprint) of every array the host setup produces for a set of
 small models covering each path of the setup -- mesh templates, forest
 refinement and 2:1 balance, DofMap numbering and hanging-node rows, the
 transpose CSR, geometry kinds and block metrics, mass, Stacey, Dirichlet and
@@ -31,22 +32,25 @@ from insem_amd import configs, mesh  # noqa: E402
 from setup_arrays import setup_arrays  # noqa: E402
 
 OUT = os.path.join(HERE, "setup_digests.json")
-# full copies of the float64 arrays (coordinates, constraint weights, rules):
-# compared within a few ulps, every other array by its digest
-OUT64 = os.path.join(HERE, "setup_float64.npz")
 FIXED_DT = 1.234e-6  # source amplitude tables are digested at this dt, 40 steps
+N_SPOTS = 32
 
 
-FULL64 = {}  # name -> float64 array, filled by digest(..., name=...)
-
-
-def digest(a, name=None):
+def digest(a):
+    """SHA-256 of the bytes, and for floating arrays a position-sensitive
+    fingerprint (sums, a cosine-weighted sum, extremes and N_SPOTS evenly
+    spaced values) that a comparison within a few ulps can use."""
     a = np.ascontiguousarray(a)
-    if name is not None and a.dtype == np.float64:
-        FULL64[name] = a.copy()
-    head = a.ravel()[:6]
-    return {"sha256": hashlib.sha256(a.tobytes()).hexdigest(), "shape": list(a.shape), "dtype": str(a.dtype),
-            "head": [float(x) for x in head]}
+    out = {"sha256": hashlib.sha256(a.tobytes()).hexdigest(), "shape": list(a.shape), "dtype": str(a.dtype),
+           "head": [float(x) for x in a.ravel()[:6]]}
+    if a.dtype.kind == "f" and a.size:
+        x = a.ravel().astype(np.float64)
+        c = np.cos(0.1 * np.arange(x.size))
+        idx = np.unique(np.linspace(0, x.size - 1, N_SPOTS).astype(np.int64))
+        out["fp"] = {"sum": float(x.sum()), "sum2": float((x * x).sum()), "wsum": float((c * x).sum()),
+                     "absmax": float(np.abs(x).max()), "min": float(x.min()), "max": float(x.max()),
+                     "spot_idx": [int(i) for i in idx], "spots": [float(x[i]) for i in idx]}
+    return out
 
 
 def _borehole(azimuthal_order, order=4):
@@ -120,11 +124,11 @@ MODELS = {
 }
 
 
-def model_digests(sim, name=""):
+def model_digests(sim):
     img = sim.build_setup()
-    out = {k: digest(v, f"{name}/{k}") for k, v in setup_arrays(img.view).items()}
+    out = {k: digest(v) for k, v in setup_arrays(img.view).items()}
     for di in range(img.n_domains):
-        out[f"d{di}.coords"] = digest(img.coordinates(di), f"{name}/d{di}.coords")
+        out[f"d{di}.coords"] = digest(img.coordinates(di))
     out["start_vectors"] = digest(img.start_vectors())
     out["amplitudes"] = digest(img.amplitudes(FIXED_DT, 40))
     out["traces"] = {"names": [list(t) for t in img.traces]}
@@ -147,7 +151,7 @@ def mesh_digests():
     out = {}
     for name, m in cases.items():
         for field in ("nodes", "cells", "material", "boundary"):
-            out[f"{name}.{field}"] = digest(np.asarray(getattr(m, field)), f"meshes/{name}.{field}")
+            out[f"{name}.{field}"] = digest(np.asarray(getattr(m, field)))
         out[f"{name}.kind"] = {"cell_type": m.cell_type, "coords": m.coords}
     return out
 
@@ -157,13 +161,13 @@ def basis_digests():
     for n in range(2, 11):
         for a, b in ((0, 0), (0, 1)):
             x, w = insem.gauss_lobatto_jacobi(n, a, b)
-            out[f"rule{n}.{a}{b}.x"] = digest(x, f"basis/rule{n}.{a}{b}.x")
-            out[f"rule{n}.{a}{b}.w"] = digest(w, f"basis/rule{n}.{a}{b}.w")
+            out[f"rule{n}.{a}{b}.x"] = digest(x)
+            out[f"rule{n}.{a}{b}.w"] = digest(w)
     return out
 
 
 def compute():
-    return {"models": {k: model_digests(f(), k) for k, f in MODELS.items()}, "meshes": mesh_digests(),
+    return {"models": {k: model_digests(f()) for k, f in MODELS.items()}, "meshes": mesh_digests(),
             "basis": basis_digests()}
 
 
@@ -171,6 +175,5 @@ if __name__ == "__main__":
     data = compute()
     with open(OUT, "w") as fh:
         json.dump(data, fh, indent=0, sort_keys=True)
-    np.savez_compressed(OUT64, **FULL64)
     print(f"wrote {OUT}: {sum(len(v) for v in data['models'].values())} model arrays, "
           f"{len(data['meshes'])} mesh arrays, {len(data['basis'])} basis arrays")
