@@ -66,7 +66,7 @@ const char* insem_host_last_error(void) { return g_err.c_str(); }
 
 int insem_basis_rule(int32_t n, int32_t alpha, int32_t beta, double* points, double* weights) {
   return guard([&] {
-    const Rule r = lobatto_jacobi_rule(n, alpha, beta);
+    const Rule r = lobatto_rule(n, alpha, beta);
     std::memcpy(points, r.points.data(), n * sizeof(double));
     std::memcpy(weights, r.weights.data(), n * sizeof(double));
   });
