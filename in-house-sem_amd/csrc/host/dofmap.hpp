@@ -1,12 +1,16 @@
-// Local-to-global point map with hanging-node constraint rows and its
-// transpose, built on the host.
+// Local-to-global point map of one medium, its hanging-node constraint rows
+// and its transpose, built on the host (the map of
+// cpp/include/insem/ops/dof_map.hpp:30-92, with the same numbering).
 //
-// Restates ops::DofMap::build (cpp/src/ops/dof_map.cpp:42-260): global
-// indices in first-touch order over the leaves (vertices, then edge interiors
-// in canonical edge direction, then element interiors), constrained slots on
-// hanging faces resolved recursively into rows of (global point, weight), and
-// the transpose CSR with contributions in ascending slot order.  The
-// ascending slot order is the assembly summation order of the hot path.
+// Numbering: elements in leaf order; an element's new edges take their
+// interior indices first (faces in order, each edge counted from its lower
+// vertex id), then its slots (j outer, i inner) give new vertices and
+// interior points theirs.  Slots on faces that lie on half of a coarser face
+// are the coarse face's interpolant at the slot: a direct index when that is
+// one point with weight 1, otherwise a constraint row of (point, weight)
+// pairs in ascending point order.  The transpose lists every point's
+// contributing slots in ascending slot order, the assembly summation order
+// of the hot path.
 #pragma once
 
 #include <cstdint>
