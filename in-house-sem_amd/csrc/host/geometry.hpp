@@ -1,9 +1,13 @@
-// Per-point geometry of the leaves and point location, on the host in double.
+// Per-point geometry of the leaves and point location, on the host in
+// double (the geometry of cpp/src/geometry/geometry.cpp:44-109 and the point
+// location of cpp/src/solver/locate.cpp:9-75).
 //
-// Restates cpp/src/geometry/geometry.cpp:44-109 (inverse metric, Jacobian,
-// Rect / Affine / Curved classification with relative tolerance 1e-10) and
-// cpp/src/solver/locate.cpp:9-75 (Newton inversion of a leaf mapping,
-// Lagrange interpolation weights and physical gradients).
+// Every leaf's map is its macro cell's map on the leaf's box; at each element
+// point it gives the coordinates, the Jacobian determinant and the inverse
+// metric.  A leaf is Rect when its Jacobian is one diagonal matrix, Affine
+// when it is one matrix, Curved otherwise (relative tolerance 1e-10).  A
+// point is located in the first candidate leaf whose map reaches it (Newton
+// in reference coordinates) and weighted by the tensor Lagrange basis.
 #pragma once
 
 #include <array>
