@@ -1,17 +1,18 @@
-// Macro-mesh and quadtree forest of the host setup.
+// Macro mesh and quadtree forest of the host setup (the mesh layer of
+// cpp/include/insem/mesh/macro_mesh.hpp:42-106 and forest.hpp:50-121).
 //
-// Restates the reference's mesh layer (cpp/include/insem/mesh/macro_mesh.hpp:42-106,
-// forest.hpp:50-121; cpp/src/mesh/macro_mesh.cpp:72-228, forest.cpp:98-261):
-// the numbering and ordering rules (edge orientation, axis-face rotation,
-// Morton leaf order, refinement child order, 2:1 balance) are kept identical
-// because the element order fixes the assembly summation order of the hot path.
+// Rules that fix the element order, and through it the assembly summation
+// order of the hot path, and so are those of the reference: cells with a
+// face on the axis are turned so that it is face 3; a tree's leaves are in
+// Morton order and a refined leaf's four children replace it in place, in
+// Morton order; 2:1 balance refines the coarser side; lattice vertices are
+// numbered in order of first appearance over the leaves' corners.
 #pragma once
 
 #include <array>
 #include <cstdint>
 #include <functional>
 #include <memory>
-#include <unordered_map>
 #include <vector>
 
 namespace insem_amd {
@@ -103,7 +104,6 @@ class Forest {
   const Leaf& leaf(int32_t l) const { return leaves_[l]; }
   std::array<double, 4> leaf_box(int32_t l) const;
   Corners corners() const;
-  int32_t find_leaf(int32_t tree, int64_t a, int64_t b) const;
   Neighbor neighbor(int32_t l, int f) const;
   double face_coordinate(int32_t from, int f, double t, int32_t to, int to_face) const;
   Forest refine(const std::vector<char>& mark) const;
@@ -113,15 +113,13 @@ class Forest {
 
  private:
   Forest(std::shared_ptr<const MacroMesh> macro, std::vector<Leaf> leaves);
-  static std::pair<uint64_t, uint64_t> key(int32_t tree, int level, int64_t i, int64_t j);
-  struct KeyHash {
-    std::size_t operator()(const std::pair<uint64_t, uint64_t>& k) const {
-      return static_cast<std::size_t>((k.first * 0x9e3779b97f4a7c15ULL) ^ (k.second + 0x632be59bd9b4e019ULL));
-    }
-  };
+  // the leaf of a tree that covers max-level cell (x, y), or -1
+  int32_t leaf_at(int32_t tree, int64_t x, int64_t y) const;
   std::shared_ptr<const MacroMesh> macro_;
   std::vector<Leaf> leaves_;
-  std::unordered_map<std::pair<uint64_t, uint64_t>, int32_t, KeyHash> lookup_;
+  // leaves are sorted by (tree, Morton code of their max-level anchor cell)
+  std::vector<uint64_t> code_;        // Morton code of each leaf's anchor
+  std::vector<int32_t> tree_first_;   // first leaf of each tree (+ end)
 };
 
 }  // namespace insem_amd
