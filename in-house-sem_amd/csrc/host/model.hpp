@@ -1,12 +1,13 @@
 // Problem description and the host-side builder of the time-loop state.
 //
-// Description types mirror cpp/include/insem/sim/description.hpp:11-141 and
-// Discretization / build_forest mirror cpp/include/insem/sim/simulation.hpp:22-48,
-// cpp/src/sim/simulation.cpp:121-185.  SetupImage is what the reference's
-// Solver constructor builds (cpp/src/solver/solver.cpp:147-568): domains with
-// their DofMap, element blocks, lumped mass, Stacey/Dirichlet/axis points,
-// the fluid-solid coupling, sources and receivers -- owned here as plain
-// arrays and exposed through the C-ABI struct insem_hip_setup.
+// The description types carry the reference's user-facing model
+// (cpp/include/insem/sim/description.hpp:11-141; Discretization and the
+// forest of simulation.hpp:22-48, simulation.cpp:121-185).  SetupImage is
+// the state the reference's Solver constructor builds
+// (cpp/src/solver/solver.cpp:147-568) -- domains with their point maps,
+// element blocks, lumped mass, Stacey/Dirichlet/axis points, fluid-solid
+// coupling, sources and receivers -- as owned plain arrays behind the C-ABI
+// struct insem_hip_setup (built in setup_image.cpp).
 #pragma once
 
 #include <array>
@@ -160,6 +161,13 @@ struct SetupImage {
   int64_t total_dofs() const;
   int64_t total_points() const;
 };
+
+// The setup image of a set-up model (setup_image.cpp).
+std::unique_ptr<SetupImage> build_setup_image(const Forest& F, const Geometry& G, const Rule& gll, int azimuthal_order,
+                                              const std::map<int, Material>& materials,
+                                              const std::map<int, Boundary>& boundaries, const TimeConfig& time,
+                                              const std::vector<Source>& sources,
+                                              const std::vector<Receiver>& receivers);
 
 class Model {
  public:

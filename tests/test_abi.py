@@ -64,9 +64,9 @@ def test_null_context_is_an_error():
 
 def test_invalid_materials_are_rejected():
     sim = insem.Simulation(grid(2, 2), insem.Discretization(order=3))
-    with pytest.raises(insem.InsemError, match="must exceed S velocity"):
+    with pytest.raises(insem.InsemError, match="must be below the P velocity"):
         sim.set_material(0, insem.Material.elastic(1.0, 2.0, 1.0))
-    with pytest.raises(insem.InsemError, match="density must be positive"):
+    with pytest.raises(insem.InsemError, match="positive density"):
         sim.set_material(0, insem.Material.acoustic(1.0, -1.0))
 
 
@@ -83,7 +83,7 @@ def test_sources_and_receivers_outside_their_medium_fail_clearly():
         sim.build_setup()
     sim.clear_receivers()
     sim.add_receiver("r1", 0.5, 0.5, ["pressure"])
-    with pytest.raises(insem.InsemError, match="cannot be recorded"):
+    with pytest.raises(insem.InsemError, match="is not recorded in elastic media"):
         sim.build_setup()
 
 
@@ -100,12 +100,12 @@ def test_non_axisymmetric_sources_on_the_axis_are_rejected():
     sim = insem.Simulation(m, insem.Discretization(order=3))
     sim.set_material(0, insem.Material.acoustic(1.5, 1.0))
     sim.add_source(insem.Source.dipole(0.0, 0.0, 1.0, 0.0, insem.ricker(5.0)))
-    with pytest.raises(insem.InsemError, match="not axisymmetric"):
+    with pytest.raises(insem.InsemError, match="azimuthal order 1, not 0"):
         sim.build_setup()
     sim1 = insem.Simulation(m, insem.Discretization(order=3, azimuthal_order=1))
     sim1.set_material(0, insem.Material.acoustic(1.5, 1.0))
     sim1.add_source(insem.Source.monopole(0.0, 0.0, insem.ricker(5.0)))
-    with pytest.raises(insem.InsemError, match="force and dipole sources only"):
+    with pytest.raises(insem.InsemError, match="not a force or a dipole"):
         sim1.build_setup()
 
 
@@ -117,5 +117,5 @@ def test_azimuthal_order_needs_axisymmetric_mesh():
 def test_unsupported_polynomial_order_is_rejected():
     sim = insem.Simulation(grid(1, 1), insem.Discretization(order=9))
     sim.set_material(0, insem.Material.acoustic(1.5, 1.0))
-    with pytest.raises(insem.InsemError, match="not supported by the kernels"):
+    with pytest.raises(insem.InsemError, match="polynomial orders 1 to 8"):
         sim.build_setup()
