@@ -41,6 +41,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "in-house-sem_amd"))
 
 HBM_PEAK_GBS = 8000.0
+OPEN_HOLE_WORKLOAD = ("axisymmetric open-hole monopole, order 4 with GLJ on the axis, 20 kHz AMR, formation radius 4 m, "
+                      "13 receivers")
 # HBM traffic of the dominant force launch, measured with rocprofv3 --pmc
 # (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of known size).
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_forces_pair_layered_800x400.txt")
@@ -319,6 +321,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-open-hole", action="store_true", help="skip the open-hole monopole leg")
     ap.add_argument("--calibrate-copy", action="store_true", help="run only a 1 GiB device copy (counter scale)")
+    ap.add_argument("--only-open-hole", action="store_true",
+                    help="run only the open-hole monopole leg and print its figures (for profiling it)")
     ap.add_argument("--batch-models", type=int, default=128, help="models of the batched leg (0: skip it)")
     ap.add_argument("--batch-plain-sample", type=int, default=24,
                     help="models of the batched leg's sequential plain comparison (the first of the same draw)")
@@ -338,6 +342,11 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.only_open_hole:
+        from insem_amd.configs import open_hole_monopole
+
+        print(json.dumps(plain_leg(open_hole_monopole(), OPEN_HOLE_WORKLOAD, args.steps, args.warmup)), flush=True)
         return
 
     import insem_amd  # noqa: F401
@@ -403,9 +412,7 @@ def main():
     if not args.no_open_hole:
         from insem_amd.configs import open_hole_monopole
 
-        result["open_hole"] = plain_leg(
-            open_hole_monopole(), "axisymmetric open-hole monopole, order 4 with GLJ on the axis, 20 kHz AMR, "
-            "formation radius 4 m, 13 receivers", args.steps, args.warmup)
+        result["open_hole"] = plain_leg(open_hole_monopole(), OPEN_HOLE_WORKLOAD, args.steps, args.warmup)
         progress(f"open-hole model: {result['open_hole']['ms_per_step']:.4f} ms/step, "
                  f"step at {result['open_hole']['step_frac']:.3f} of the HBM roofline")
     if args.batch_models > 0:
