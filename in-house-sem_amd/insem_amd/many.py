@@ -8,9 +8,11 @@ models, and runs all batches side by side, each on its own stream: a
 batch runs until its longest model ends, so batches of similar models
 waste few steps, and many small batches side by side keep the device busy
 while the longer ones finish.  The models' host setups (which make no
-device calls) are built on worker threads while the batches already
-created run; every device call is made from the calling thread.  Each
-model's traces are bitwise those of running it alone.
+device calls) are built on worker threads, all of them before the first
+batch is planned (the plan needs every model's size); each batch starts
+stepping as soon as it is created, while the next ones are set up.  Every
+device call is made from the calling thread.  Each model's traces are
+bitwise those of running it alone.
 """
 
 from __future__ import annotations
