@@ -267,11 +267,13 @@ def served_leg(n_requests, rate, sweep, slots=4, batch=8, seed=7):
     """A Poisson stream of mixed requests (60 % cased-hole dipole, 30 %
     open-hole monopole at 3-8 ms; 10 % 2D layered coupled at 1-4 M nodes for
     0.5-1 s) served by the continuous-batching loop at each arrival rate of
-    the sweep (the same requests, inter-arrival times scaled), and at `rate`
-    also one request at a time on the plain path (one slot, no batching).
+    the sweep (the same requests, inter-arrival times scaled; the layered
+    requests in a slot class of their own), and at `rate` also one request
+    at a time on the plain path (one slot, no batching, no large slot).
     Latency = arrival to traces (and STC for boreholes) ready; the rate at
     which solves/s stops following the arrival rate is the loop's
-    saturation."""
+    saturation, compared with the one-at-a-time rate at a saturating
+    arrival rate (the device never idles there either)."""
     import numpy as np
 
     from insem_amd import serve
@@ -281,26 +283,35 @@ def served_leg(n_requests, rate, sweep, slots=4, batch=8, seed=7):
 
     stc = dict(slowness=np.linspace(100e-6, 1500e-6, 141), window=5e-4, times=np.linspace(0.0, 2.5e-3, 51))
 
-    def run(r, slots, batch):
+    def run(r, slots, batch, large_slots):
         stream = [(float(t) / r, build(k, sd)) for t, k, sd in zip(unit_arrivals, kinds, seeds)]
-        srv = serve.Server(max_slots=slots, max_batch=batch, stc=stc)
+        srv = serve.Server(max_slots=slots, max_batch=batch, stc=stc, max_large_slots=large_slots)
         t0 = time.perf_counter()
         reqs = srv.serve(stream)
-        return serve.summary(reqs, time.perf_counter() - t0)
+        out = serve.summary(reqs, time.perf_counter() - t0)
+        for q in reqs:
+            if not q.ok:
+                progress(f"served leg: request {q.rid} ({kinds[q.rid]}) failed: {q.error}")
+        lat = {k: [x.latency * 1e3 for x in reqs if x.ok and kinds[x.rid] == k] for k in ("cased", "open", "layered")}
+        out["latency_p50_ms_by_kind"] = {k: float(np.percentile(v, 50)) for k, v in lat.items() if v}
+        return out
 
     out = {"sweep": []}
     for r in sorted(set(sweep) | {rate}):
-        res = dict(rate=r, **run(r, slots, batch))
+        res = dict(rate=r, **run(r, slots, batch, 1))
         out["sweep"].append(res)
-        progress(f"served leg: {res['solves_per_s']:.3f} solves/s at {r:g}/s")
+        progress(f"served leg: {res['solves_per_s']:.3f} solves/s at {r:g}/s "
+                 f"(p50 {res['latency_p50_ms']:.0f} ms, p99 {res['latency_p99_ms']:.0f} ms)")
         if r == rate:
             out["served"] = res
-    out["one_at_a_time"] = dict(rate=rate, **run(rate, 1, 1))
+    out["one_at_a_time"] = dict(rate=rate, **run(rate, 1, 1, 0))
     progress(f"served leg: one at a time {out['one_at_a_time']['solves_per_s']:.3f} solves/s")
     out["saturation_solves_per_s"] = max(x["solves_per_s"] for x in out["sweep"])
+    out["vs_one_at_a_time"] = out["saturation_solves_per_s"] / out["one_at_a_time"]["solves_per_s"]
     out["sample"] = (f"{n_requests} requests, Poisson arrivals (seed {seed}) at rates {sorted(set(sweep) | {rate})}/s: "
                      f"{int((kinds == 'cased').sum())} cased-hole, {int((kinds == 'open').sum())} open-hole, "
-                     f"{int((kinds == 'layered').sum())} layered; served: {slots} slots, batches of up to {batch}; "
+                     f"{int((kinds == 'layered').sum())} layered; served: {slots} slots, batches of up to {batch}, "
+                     f"plus one slot for layered requests; "
                      f"one at a time at {rate:g}/s")
     return out
 
@@ -334,10 +345,10 @@ def main():
     ap.add_argument("--batch-duration", type=float, default=5e-3, help="simulated time of each batched model (s)")
     ap.add_argument("--batch-size", type=int, default=5,
                     help="models per batch (each polynomial order's models, sorted by size, in batches of up to this)")
-    ap.add_argument("--serve-requests", type=int, default=16, help="requests of the served leg (0: skip it)")
-    ap.add_argument("--serve-rate", type=float, default=2.0,
+    ap.add_argument("--serve-requests", type=int, default=100, help="requests of the served leg (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=8.0,
                     help="Poisson arrival rate of the served leg's comparison with one-at-a-time service (1/s)")
-    ap.add_argument("--serve-sweep", default="1,2,4", help="arrival rates (1/s) the served loop runs at")
+    ap.add_argument("--serve-sweep", default="2,4,8", help="arrival rates (1/s) the served loop runs at")
     ap.add_argument("--serve-slots", type=int, default=4, help="batches the served loop keeps on the device at once")
     ap.add_argument("--serve-batch", type=int, default=8, help="requests per batch of the served loop")
     args = ap.parse_args()

@@ -168,39 +168,86 @@ BatchImage union_of(const std::vector<const SetupImage*>& models) {
   return out;
 }
 
+PowerJob::PowerJob(const SetupImage& model) {
+  check_hip(insem_hip_create(&model.view, &ctx_));
+  try {
+    const std::vector<float> start = model.power_start_vectors();
+    check_hip(insem_hip_lambda_start(ctx_, start.data()));
+  } catch (...) {
+    insem_hip_destroy(ctx_);
+    ctx_ = nullptr;
+    throw;
+  }
+}
+
+PowerJob::~PowerJob() {
+  if (ctx_) insem_hip_destroy(ctx_);
+}
+
+bool PowerJob::ready() const {
+  if (!ctx_) return true;
+  int32_t idle = 0;
+  check_hip(insem_hip_query(ctx_, &idle));
+  return idle != 0;
+}
+
+double PowerJob::lambda() {
+  if (ctx_) {
+    insem_hip_ctx* c = ctx_;
+    ctx_ = nullptr;
+    const int rc = insem_hip_lambda_result(c, &lambda_);
+    insem_hip_destroy(c);
+    check_hip(rc);
+  }
+  return lambda_;
+}
+
 BatchSolver::BatchSolver(std::vector<std::unique_ptr<SetupImage>> models) : models_(std::move(models)) {
   INSEM_REQUIRE(!models_.empty(), "a batch needs at least one model");
-  // Each model's own power iteration on a short-lived context of it alone.
-  // A window of them is queued before the first result is read, so that
-  // the iterations of different models overlap on the device while the host
-  // queues the next ones.
   for (const auto& m : models_) INSEM_REQUIRE(m, "null model in a batch");
   // Each model's own power iteration on a short-lived context of it alone,
   // all on the calling thread.  A window of them is queued before the first
   // result is read, so that the iterations of different models overlap on
   // the device while the host builds the next contexts.
   constexpr std::size_t kWindow = 16;
+  std::vector<double> lambdas;
   for (std::size_t w0 = 0; w0 < models_.size(); w0 += kWindow) {
     const std::size_t w1 = std::min(models_.size(), w0 + kWindow);
-    std::vector<insem_hip_ctx*> ctx(w1 - w0, nullptr);
-    try {
-      for (std::size_t i = w0; i < w1; ++i) {
-        check_hip(insem_hip_create(&models_[i]->view, &ctx[i - w0]));
-        const std::vector<float> start = models_[i]->power_start_vectors();
-        check_hip(insem_hip_lambda_start(ctx[i - w0], start.data()));
-      }
-      for (std::size_t i = w0; i < w1; ++i) {
-        double lambda = 0.0;
-        check_hip(insem_hip_lambda_result(ctx[i - w0], &lambda));
-        time_.push_back(model_time(models_[i]->time, lambda));
-        max_steps_ = std::max(max_steps_, time_.back().n_steps);
-      }
-    } catch (...) {
-      for (insem_hip_ctx* c : ctx) insem_hip_destroy(c);
-      throw;
-    }
-    for (insem_hip_ctx* c : ctx) insem_hip_destroy(c);
+    std::vector<std::unique_ptr<PowerJob>> jobs;
+    for (std::size_t i = w0; i < w1; ++i) jobs.push_back(std::make_unique<PowerJob>(*models_[i]));
+    for (auto& j : jobs) lambdas.push_back(j->lambda());
   }
+  start(lambdas);
+}
+
+BatchSolver::BatchSolver(std::vector<std::unique_ptr<SetupImage>> models, const std::vector<double>& lambdas)
+    : models_(std::move(models)) {
+  INSEM_REQUIRE(!models_.empty(), "a batch needs at least one model");
+  for (const auto& m : models_) INSEM_REQUIRE(m, "null model in a batch");
+  INSEM_REQUIRE(lambdas.size() == models_.size(),
+                "a batch of " << models_.size() << " models needs as many lambda_max values, not " << lambdas.size());
+  start(lambdas);
+}
+
+void BatchSolver::start(const std::vector<double>& lambdas) {
+  const std::size_t n = models_.size();
+  std::vector<ModelTime> own;
+  for (std::size_t i = 0; i < n; ++i) {
+    own.push_back(model_time(models_[i]->time, lambdas[i]));
+    max_steps_ = std::max(max_steps_, own.back().n_steps);
+  }
+  // union order: longest first (ties: the caller's order)
+  std::vector<int> order(n);
+  for (std::size_t i = 0; i < n; ++i) order[i] = static_cast<int>(i);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return own[a].n_steps > own[b].n_steps; });
+  std::vector<std::unique_ptr<SetupImage>> sorted;
+  slot_.assign(n, 0);
+  for (std::size_t u = 0; u < n; ++u) {
+    sorted.push_back(std::move(models_[order[u]]));
+    time_.push_back(own[order[u]]);
+    slot_[order[u]] = static_cast<int>(u);
+  }
+  models_ = std::move(sorted);
   std::vector<const SetupImage*> ptrs;
   for (const auto& m : models_) ptrs.push_back(m.get());
   union_ = union_of(ptrs);
@@ -249,7 +296,7 @@ int BatchSolver::n_samples(int m) const {
   int32_t ns = 0;
   check_hip(insem_hip_n_samples(ctx_, &ns));
   if (ns == 0) return 0;
-  const int own = time_.at(m).n_steps / union_.image->time.record_every + 1;
+  const int own = n_steps(m) / union_.image->time.record_every + 1;
   return std::min(static_cast<int>(ns), own);
 }
 
@@ -265,10 +312,11 @@ std::vector<Solver::Trace> BatchSolver::traces(int m) const {
   const int keep = n_samples(m);
   std::vector<Solver::Trace> out;
   std::size_t o = 0;
-  const int r0 = union_.receiver_offset.at(m), r1 = union_.receiver_offset.at(m + 1);
+  const int u = slot(m);
+  const int r0 = union_.receiver_offset.at(u), r1 = union_.receiver_offset.at(u + 1);
   for (int ri = 0; ri < static_cast<int>(U.receivers.size()); ++ri) {
     const auto& r = U.receivers[ri];
-    const auto& own = models_[m]->receivers;
+    const auto& own = models_[u]->receivers;
     for (std::size_t q = 0; q < r.quantities.size(); ++q) {
       const std::size_t n = static_cast<std::size_t>(r.ncomp[q]) * ns;
       if (ri >= r0 && ri < r1) {

@@ -30,21 +30,45 @@ struct BatchImage {
 // azimuthal order, the media present and record_every.
 BatchImage union_of(const std::vector<const SetupImage*>& models);
 
+// One model's power iteration (lambda_max of M^-1 K, as a Solver of the
+// model alone runs it) queued on a short-lived context of that model: the
+// constructor uploads the model and queues the iterations without waiting,
+// ready() asks whether they have finished, lambda() waits for the result
+// and frees the context.  The model is copied to the device by the
+// constructor and not read afterwards.
+class PowerJob {
+ public:
+  explicit PowerJob(const SetupImage& model);
+  ~PowerJob();
+  PowerJob(const PowerJob&) = delete;
+  PowerJob& operator=(const PowerJob&) = delete;
+  bool ready() const;
+  double lambda();
+
+ private:
+  insem_hip_ctx* ctx_ = nullptr;
+  double lambda_ = 0.0;
+};
+
 // Many models stepped as one union on the device (insem_hip_create_batch).
 // Each model's Δt comes from its own power iteration (a short-lived
-// single-model context), as a Solver of that model alone would get it.
+// single-model context), as a Solver of that model alone would get it; the
+// second constructor takes those lambda_max values already computed (by
+// PowerJobs started earlier), so that it makes no power iteration itself.
 class BatchSolver {
  public:
   explicit BatchSolver(std::vector<std::unique_ptr<SetupImage>> models);
+  BatchSolver(std::vector<std::unique_ptr<SetupImage>> models, const std::vector<double>& lambdas);
   ~BatchSolver();
   BatchSolver(const BatchSolver&) = delete;
   BatchSolver& operator=(const BatchSolver&) = delete;
 
   int n_models() const { return static_cast<int>(models_.size()); }
-  const SetupImage& model(int m) const { return *models_.at(m); }
-  double dt(int m) const { return time_.at(m).dt; }
-  double stable_dt(int m) const { return time_.at(m).stable_dt; }
-  int n_steps(int m) const { return time_.at(m).n_steps; }
+  // m: the model's place in the list the batch was made from.
+  const SetupImage& model(int m) const { return *models_.at(slot(m)); }
+  double dt(int m) const { return time_.at(slot(m)).dt; }
+  double stable_dt(int m) const { return time_.at(slot(m)).stable_dt; }
+  int n_steps(int m) const { return time_.at(slot(m)).n_steps; }
   int max_steps() const { return max_steps_; }
   int step() const;
   void advance(int n);
@@ -57,7 +81,15 @@ class BatchSolver {
   int64_t footprint() const;
 
  private:
+  void start(const std::vector<double>& lambdas);
+  int slot(int m) const { return slot_.at(m); }
+
+  // The models in union order: by step count, longest first, so that the
+  // work of the models still running is a leading part of every launch
+  // (the backend then launches only that part); slot_[m] is the union
+  // place of the caller's model m.
   std::vector<std::unique_ptr<SetupImage>> models_;
+  std::vector<int> slot_;
   BatchImage union_;
   std::vector<ModelTime> time_;
   int max_steps_ = 0;

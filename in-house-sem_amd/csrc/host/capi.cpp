@@ -460,7 +460,38 @@ void* insem_batch_new(void* const* images, int32_t n) {
   });
 }
 
+// As insem_batch_new, with each model's lambda_max given (from a power job
+// of it alone): makes no power iteration.
+void* insem_batch_new_lambdas(void* const* images, const double* lambdas, int32_t n) {
+  return guard_ptr([&]() -> void* {
+    INSEM_REQUIRE(lambdas && n >= 0, "null lambda_max values");
+    std::vector<std::unique_ptr<SetupImage>> v;
+    for (int32_t i = 0; i < n; ++i) v.emplace_back(static_cast<SetupImage*>(images[i]));
+    return new BatchSolver(std::move(v), std::vector<double>(lambdas, lambdas + n));
+  });
+}
+
 void insem_batch_free(void* h) { delete static_cast<BatchSolver*>(h); }
+
+// ---------------------------------------------------------------- power jobs
+
+// Uploads the model to a context of its own and queues its power
+// iteration; the image stays the caller's and may be freed afterwards.
+void* insem_power_start(void* image) {
+  return guard_ptr([&]() -> void* { return new PowerJob(*static_cast<const SetupImage*>(image)); });
+}
+
+// *ready = 1 when the iteration has finished (lambda then returns at once).
+int insem_power_ready(void* job, int32_t* ready) {
+  return guard([&] { *ready = static_cast<PowerJob*>(job)->ready() ? 1 : 0; });
+}
+
+// Waits for lambda_max; the job's context is freed, the job stays readable.
+int insem_power_lambda(void* job, double* lambda) {
+  return guard([&] { *lambda = static_cast<PowerJob*>(job)->lambda(); });
+}
+
+void insem_power_free(void* job) { delete static_cast<PowerJob*>(job); }
 
 void* insem_batch_context(void* h) { return static_cast<BatchSolver*>(h)->context(); }
 

@@ -141,7 +141,12 @@ def host():
         d("insem_solver_n_energy", c_int, c_void)
         d("insem_solver_energy", c_int, c_void, P(c_double))
         d("insem_batch_new", c_void, P(c_void), c_int)
+        d("insem_batch_new_lambdas", c_void, P(c_void), P(c_double), c_int)
         d("insem_batch_free", None, c_void)
+        d("insem_power_start", c_void, c_void)
+        d("insem_power_ready", c_int, c_void, P(c_int))
+        d("insem_power_lambda", c_int, c_void, P(c_double))
+        d("insem_power_free", None, c_void)
         d("insem_batch_context", c_void, c_void)
         d("insem_batch_model_image", c_void, c_void, c_int)
         d("insem_batch_info", c_int, c_void, c_int, P(c_double), P(c_int))

@@ -438,6 +438,36 @@ class Solver:
 # ------------------------------------------------------------------ batch
 
 
+class PowerJob:
+    """A model's power iteration (lambda_max of M^-1 K, the one its Solver
+    runs for the stable time step) queued on a short-lived device context of
+    the model alone, without waiting (no counterpart in the reference, which
+    estimates the step synchronously in Solver::estimate_stable_dt,
+    solver.cpp:583-667).  ready() does not block; lambda_max() waits."""
+
+    def __init__(self, image: "SetupImage"):
+        self._h = check_ptr(_lib.host().insem_power_start(image._h))
+
+    def ready(self) -> bool:
+        flag = ctypes.c_int32()
+        check(_lib.host().insem_power_ready(self._h, ctypes.byref(flag)))
+        return bool(flag.value)
+
+    def lambda_max(self) -> float:
+        lam = ctypes.c_double()
+        check(_lib.host().insem_power_lambda(self._h, ctypes.byref(lam)))
+        return lam.value
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.host().insem_power_free(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 class BatchSolver:
     """Many independent models stepped as one union on the device (no
     counterpart in the reference, which runs one model per Solver).  Each
@@ -446,7 +476,9 @@ class BatchSolver:
     alone.  Models must share the polynomial order, coordinate system,
     azimuthal order, media and record cadence."""
 
-    def __init__(self, models):
+    def __init__(self, models, lambdas: Optional[Sequence[float]] = None):
+        """lambdas: each model's lambda_max from a PowerJob of it (then the
+        batch makes no power iteration of its own), or None."""
         lib = _lib.host()
         models = list(models)
         sims = [m for m in models if isinstance(m, Simulation)]
@@ -464,7 +496,13 @@ class BatchSolver:
         handles = (ctypes.c_void_p * len(images))(*[im._h for im in images])
         for im in images:
             im._owned = False  # the batch takes the images over
-        self._handle = check_ptr(lib.insem_batch_new(handles, len(images)))
+        if lambdas is None:
+            self._handle = check_ptr(lib.insem_batch_new(handles, len(images)))
+        else:
+            lam = np.ascontiguousarray(lambdas, dtype=np.float64)
+            if lam.shape != (len(images),):
+                raise InsemError(f"a batch of {len(images)} models needs as many lambda_max values, not {lam.size}")
+            self._handle = check_ptr(lib.insem_batch_new_lambdas(handles, _dptr(lam), len(images)))
         for im in images:
             im._h = None
         self.models = [SetupImage(lib.insem_batch_model_image(self._handle, m), owned=False)

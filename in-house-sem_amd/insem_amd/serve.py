@@ -1,21 +1,37 @@
 """A continuous-batching request loop on one GPU (no counterpart in the
 reference, which runs one model per process).
 
-Requests (a Simulation each) arrive over time.  The loop keeps up to
-`max_slots` batches on the device at once, each on its own stream: when a
-slot is free and the device memory allows, the queued requests that can
-share a batch with the oldest one (same polynomial order, coordinate system,
-azimuthal order, media and record cadence; at most `max_batch`) are admitted
-together as one BatchSolver; requests that arrive while others run wait
-only for a slot, not for the others to finish.  Every slot is kept busy by
-queueing its next chunk of steps as soon as its stream is idle; a finished
-batch returns each request's traces (bitwise those of the request run
-alone) and, for borehole arrays, their slowness-time coherence.
+Requests (a Simulation each) arrive over time and go through four stages:
 
-Host work that does not touch the device (building a request's setup on
-arrival, the coherence of finished traces) runs on worker threads, so the
-loop keeps feeding the device meanwhile; every device call stays on the
-loop's thread.
+1. setup: the request's host setup is built on a worker thread as soon as
+   it arrives;
+2. time step: its power iteration (the lambda_max that fixes its Δt and
+   step count, as the request run alone would get them) is queued on the
+   device on a short-lived context of its own, and the loop polls it
+   without waiting, so that the running batches keep getting their steps;
+3. admission: requests whose time step is known wait for a slot.  Requests
+   of at least `large_dofs` degrees of freedom (the 2D layered models) have
+   a slot class of their own (`max_large_slots` slots, one request each),
+   so that they neither hold up nor wait for the borehole batches.  When a
+   small slot is free, the waiting small requests that can share a batch
+   with the oldest one (same polynomial order, coordinate system, azimuthal
+   order, media and record cadence; at most `max_batch`) are admitted
+   together as one BatchSolver, within the device-memory budget;
+4. stepping: each slot is kept busy by queueing its next chunk of steps as
+   soon as its stream is idle.  A request's traces are returned as soon as
+   its own step count is reached (bitwise those of the request run alone;
+   with their slowness-time coherence for borehole arrays), while the rest
+   of its batch runs on.
+
+Failures end one request, not the loop: a setup or power iteration that
+raises, a request larger than the memory budget (refused), traces that are
+not finite (a model that diverged) and a request still unfinished `timeout`
+seconds after its arrival (its batch is stopped when none of its requests
+is left) all set the request's `error` and its `done` time.
+
+Host work that does not touch the device (setups, the coherence of finished
+traces) runs on worker threads; every device call stays on the loop's
+thread.
 """
 
 from __future__ import annotations
@@ -30,7 +46,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _lib, post
-from .core import BatchSolver, SetupImage, Simulation
+from .core import BatchSolver, PowerJob, SetupImage, Simulation
 
 
 @dataclass
@@ -41,16 +57,24 @@ class Request:
     image: Optional[SetupImage] = None
     key: Tuple = ()
     footprint: int = 0                  # estimated device bytes
+    n_dofs: int = 0
+    large: bool = False
+    lam: float = float("nan")           # lambda_max of its power iteration
     admitted: float = float("nan")
     done: float = float("nan")
     dt: float = float("nan")
     n_steps: int = 0
     traces: Dict[Tuple[str, str], np.ndarray] = field(default_factory=dict)
     stc: Optional[post.STC] = None
+    error: Optional[str] = None
 
     @property
     def latency(self) -> float:
         return self.done - self.arrival
+
+    @property
+    def ok(self) -> bool:
+        return self.error is None
 
 
 def batch_key(img: SetupImage) -> Tuple:
@@ -86,32 +110,71 @@ class _Slot:
     batch: BatchSolver
     requests: List[Request]
     footprint: int
+    large: bool
+    open: List[int] = field(default_factory=list)   # models whose traces are still to be returned
 
     def idle(self) -> bool:
         return self.batch.idle()
 
 
+_FAILED = object()
+
+
 class Server:
     """Serves a stream of requests; see the module docstring.
 
-    hbm_budget: device bytes the served models may hold together (default:
-    90 % of the device memory free at start).  stc: keyword arguments of
-    post.stc for each borehole request's first quantity over its receivers
-    (slowness grid, window, ...; offsets are the receivers' z), or None."""
+    hbm_budget: device bytes the served models (and the contexts of running
+    power iterations) may hold together (default: 90 % of the device memory
+    free at start).  stc: keyword arguments of post.stc for each borehole
+    request's first quantity over its receivers (slowness grid, window, ...;
+    offsets are the receivers' z), or None.  timeout: seconds after its
+    arrival by which a request must have finished, or None."""
 
     def __init__(self, max_slots: int = 4, max_batch: int = 16, chunk: int = 1024,
-                 hbm_budget: Optional[int] = None, stc: Optional[dict] = None):
+                 hbm_budget: Optional[int] = None, stc: Optional[dict] = None, max_large_slots: int = 1,
+                 large_dofs: int = 2_000_000, max_power_jobs: int = 4, timeout: Optional[float] = None,
+                 workers: int = 4):
+        if max_slots < 1 or max_batch < 1 or chunk < 1 or max_large_slots < 0 or max_power_jobs < 1:
+            raise ValueError("slots, batch size, chunk and power jobs must be positive")
         self.max_slots = max_slots
         self.max_batch = max_batch
         self.chunk = chunk
         self.hbm_budget = int(0.9 * device_memory()[0]) if hbm_budget is None else int(hbm_budget)
         self.stc = stc
+        # with no large slots, large requests go to the ordinary ones
+        self.max_large_slots = max_large_slots
+        self.large_dofs = large_dofs if max_large_slots > 0 else None
+        self.max_power_jobs = max_power_jobs
+        self.timeout = timeout
+        self.workers = workers
 
-    def _admit(self, queue: deque, now: float, used: int) -> Optional[_Slot]:
+    # The device-facing steps, one method each (a test replaces them).
+    def _setup(self, r: Request) -> SetupImage:
+        return r.sim.build_setup()
+
+    def _power_job(self, img: SetupImage):
+        return PowerJob(img)
+
+    def _batch(self, images: List[SetupImage], lambdas: List[float]):
+        return BatchSolver(images, lambdas)
+
+    def _prepare(self, r: Request) -> Request:
+        r.image = self._setup(r)
+        r.key = batch_key(r.image)
+        r.footprint = footprint_estimate(r.image)
+        r.n_dofs = r.image.n_dofs
+        r.large = self.large_dofs is not None and r.n_dofs >= self.large_dofs
+        return r
+
+    def _admit(self, queue: deque, large: bool, t0: float, used: int):
+        """A batch of the waiting requests of one class; None when none fits
+        the memory left, _FAILED when the batch could not be set up (its
+        requests then fail)."""
         head = queue[0]
+        limit = 1 if large else self.max_batch
         group, need = [], 0
         for r in list(queue):
-            if r.key != head.key or len(group) == self.max_batch:
+            if r.key != head.key or len(group) == limit:
                 continue
             if used + need + r.footprint > self.hbm_budget:
                 break
@@ -119,16 +182,22 @@ class Server:
             need += r.footprint
         if not group:
             return None
+        now = time.perf_counter() - t0
         for r in group:
             queue.remove(r)
             r.admitted = now
         images = [r.image for r in group]
         for r in group:
             r.image = None  # the batch takes the images over
-        b = BatchSolver(images)
+        try:
+            b = self._batch(images, [r.lam for r in group])
+        except Exception as e:  # noqa: BLE001 - the batch's requests fail, the loop goes on
+            for r in group:
+                self._fail(r, f"batch set-up failed: {e}", t0)
+            return _FAILED
         for m, r in enumerate(group):
             r.dt, r.n_steps = b.dt(m), b.n_steps(m)
-        return _Slot(b, group, need)
+        return _Slot(b, group, need, large, list(range(len(group))))
 
     def _coherence(self, r: Request, step: float) -> None:
         q = r.sim.receivers[0].quantities[0]
@@ -137,77 +206,141 @@ class Server:
         z = np.array([rec.position[1] for rec in recs])
         r.stc = post.stc(x, z, step, **self.stc)
 
-    def _finish(self, slot: _Slot, pool: ThreadPoolExecutor, t0: float) -> List[Future]:
-        """Reads the traces of a finished batch; the coherence of borehole
-        arrays goes to the workers.  Returns their futures."""
-        b = slot.batch
-        jobs = []
-        for m, r in enumerate(slot.requests):
-            r.traces = b.traces(m)
-            if self.stc is not None and b.models[m].axisymmetric and len(r.sim.receivers) > 1:
-                step = b.models[m].time.record_every * r.dt
+    def _deliver(self, slot: _Slot, m: int, pool: ThreadPoolExecutor, t0: float) -> Optional[Future]:
+        """Returns model m's traces (its steps are done and the slot idle);
+        the coherence of a borehole array goes to the workers."""
+        b, r = slot.batch, slot.requests[m]
+        if r.error is not None:
+            return None
+        r.traces = b.traces(m)
+        if not all(np.isfinite(v).all() for v in r.traces.values()):
+            r.error = "non-finite traces: the model diverged"
+            r.done = time.perf_counter() - t0
+            return None
+        if self.stc is not None and b.models[m].axisymmetric and len(r.sim.receivers) > 1:
+            step = b.models[m].time.record_every * r.dt
 
-                def job(r=r, step=step):
-                    self._coherence(r, step)
-                    r.done = time.perf_counter() - t0
-
-                jobs.append(pool.submit(job))
-            else:
+            def job(r=r, step=step):
+                self._coherence(r, step)
                 r.done = time.perf_counter() - t0
-        b.close()
-        return jobs
 
-    @staticmethod
-    def _prepare(r: Request) -> Request:
-        r.image = r.sim.build_setup()
-        r.key = batch_key(r.image)
-        r.footprint = footprint_estimate(r.image)
-        return r
+            return pool.submit(job)
+        r.done = time.perf_counter() - t0
+        return None
+
+    def _fail(self, r: Request, msg: str, t0: float) -> None:
+        if r.error is None:
+            r.error = msg
+            r.done = time.perf_counter() - t0
 
     def serve(self, arrivals: Sequence[Tuple[float, Simulation]]) -> List[Request]:
         """Runs the stream (arrival times in s from now) to completion; returns
         the requests in arrival order with their timings and results."""
         reqs = [Request(i, sim, float(t)) for i, (t, sim) in enumerate(sorted(arrivals, key=lambda a: a[0]))]
         pending = deque(reqs)
-        building: deque = deque()   # futures of setups in arrival order
-        queue: deque = deque()
+        building: deque = deque()   # (request, future of its setup) in arrival order
+        timing: deque = deque()     # requests waiting for a power-iteration slot
+        power: List[Tuple[Request, object]] = []
+        queues = {False: deque(), True: deque()}   # waiting for admission, by slot class
         slots: List[_Slot] = []
         jobs: List[Future] = []
         used = 0
         t0 = time.perf_counter()
-        with ThreadPoolExecutor(max_workers=4) as pool:
-            while pending or building or queue or slots or jobs:
+
+        def expired(r: Request, now: float) -> bool:
+            return self.timeout is not None and r.error is None and now - r.arrival > self.timeout
+
+        with ThreadPoolExecutor(max_workers=max(1, self.workers)) as pool:
+            while pending or building or timing or power or queues[False] or queues[True] or slots or jobs:
                 now = time.perf_counter() - t0
-                while pending and pending[0].arrival <= now:
-                    building.append(pool.submit(self._prepare, pending.popleft()))
-                while building and building[0].done():
-                    r = building.popleft().result()
-                    if r.footprint > self.hbm_budget:
-                        raise MemoryError(f"request {r.rid} needs about {r.footprint} device bytes, "
-                                          f"more than the budget of {self.hbm_budget}")
-                    queue.append(r)
-                while queue and len(slots) < self.max_slots:
-                    slot = self._admit(queue, time.perf_counter() - t0, used)
-                    if slot is None:
-                        break
-                    slots.append(slot)
-                    used += slot.footprint
                 busy = False
+                while pending and pending[0].arrival <= now:
+                    r = pending.popleft()
+                    building.append((r, pool.submit(self._prepare, r)))
+                # 1. setups, in arrival order
+                while building and building[0][1].done():
+                    r, f = building.popleft()
+                    try:
+                        f.result()
+                    except Exception as e:  # noqa: BLE001 - the request fails, the loop goes on
+                        self._fail(r, f"setup failed: {e}", t0)
+                        continue
+                    if r.footprint > self.hbm_budget:
+                        self._fail(r, f"refused: needs about {r.footprint} device bytes, more than the budget of "
+                                      f"{self.hbm_budget}", t0)
+                        r.image = None
+                        continue
+                    timing.append(r)
+                # 2. power iterations: started within the budget, polled without waiting
+                while timing and len(power) < self.max_power_jobs and used + timing[0].footprint <= self.hbm_budget:
+                    r = timing.popleft()
+                    try:
+                        power.append((r, self._power_job(r.image)))
+                        used += r.footprint
+                    except Exception as e:  # noqa: BLE001
+                        self._fail(r, f"power iteration failed: {e}", t0)
+                        r.image = None
+                    busy = True
+                for r, job in list(power):
+                    if not job.ready():
+                        continue
+                    power.remove((r, job))
+                    used -= r.footprint
+                    try:
+                        r.lam = job.lambda_max()
+                    except Exception as e:  # noqa: BLE001
+                        self._fail(r, f"power iteration failed: {e}", t0)
+                        r.image = None
+                        continue
+                    finally:
+                        job.close()
+                    queues[r.large].append(r)
+                    busy = True
+                # requests past their deadline before admission
+                for q in queues.values():
+                    for r in [r for r in q if expired(r, now)]:
+                        q.remove(r)
+                        r.image = None
+                        self._fail(r, f"timed out after {self.timeout:g} s waiting for a slot", t0)
+                # 3. admission, per slot class
+                for large, cap in ((True, self.max_large_slots), (False, self.max_slots)):
+                    q = queues[large]
+                    while q and sum(s.large == large for s in slots) < cap:
+                        slot = self._admit(q, large, t0, used)
+                        if slot is _FAILED:
+                            continue
+                        if slot is None:
+                            break
+                        slots.append(slot)
+                        used += slot.footprint
+                        busy = True
+                # 4. stepping and results
                 for slot in list(slots):
+                    if not slot.idle():
+                        continue
                     b = slot.batch
-                    if b.step >= b.max_steps:
-                        if slot.idle():
-                            jobs += self._finish(slot, pool, t0)
-                            slots.remove(slot)
-                            used -= slot.footprint
-                    elif slot.idle():
+                    step = b.step
+                    for m in [m for m in slot.open if slot.requests[m].n_steps <= step]:
+                        slot.open.remove(m)
+                        j = self._deliver(slot, m, pool, t0)
+                        if j is not None:
+                            jobs.append(j)
+                    for m in [m for m in slot.open if expired(slot.requests[m], now)]:
+                        slot.open.remove(m)
+                        self._fail(slot.requests[m], f"timed out after {self.timeout:g} s", t0)
+                    if not slot.open:
+                        b.close()
+                        slots.remove(slot)
+                        used -= slot.footprint
+                    else:
                         b.advance(self.chunk)
                         busy = True
                 for j in [j for j in jobs if j.done()]:
                     j.result()
                     jobs.remove(j)
                 if not busy:
-                    if not slots and not queue and not building and not jobs and pending:
+                    idle = not (building or timing or power or queues[False] or queues[True] or slots or jobs)
+                    if idle and pending:
                         time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
                     else:
                         time.sleep(2e-4)
@@ -215,9 +348,11 @@ class Server:
 
 
 def summary(reqs: Sequence[Request], wall: float) -> dict:
-    """Solves/s and latency percentiles (ms) of a served stream."""
-    lat = np.array([r.latency for r in reqs]) * 1e3
-    wait = np.array([r.admitted - r.arrival for r in reqs]) * 1e3
-    return {"solves": len(reqs), "solves_per_s": len(reqs) / wall, "latency_p50_ms": float(np.percentile(lat, 50)),
-            "latency_p99_ms": float(np.percentile(lat, 99)), "queue_wait_p50_ms": float(np.percentile(wait, 50)),
-            "queue_wait_p99_ms": float(np.percentile(wait, 99))}
+    """Solves/s and latency percentiles (ms) of a served stream's finished
+    requests, and the count of failed ones."""
+    ok = [r for r in reqs if r.ok]
+    lat = np.array([r.latency for r in ok] or [np.nan]) * 1e3
+    wait = np.array([r.admitted - r.arrival for r in ok] or [np.nan]) * 1e3
+    return {"solves": len(ok), "failed": len(reqs) - len(ok), "solves_per_s": len(ok) / wall,
+            "latency_p50_ms": float(np.percentile(lat, 50)), "latency_p99_ms": float(np.percentile(lat, 99)),
+            "queue_wait_p50_ms": float(np.percentile(wait, 50)), "queue_wait_p99_ms": float(np.percentile(wait, 99))}

@@ -20,7 +20,7 @@ def make(kind, seed):
         return cased_hole(rng, **small)
     if kind == "open":
         return open_hole(rng, **small)
-    return layered_coupled(nx=24, nz_fluid=3, nz_solid=(5, 6), duration=0.01 + 0.01 * (seed % 3), n_receivers=5,
+    return layered_coupled(nx=36, nz_fluid=3, nz_solid=(5, 6), duration=0.01 + 0.01 * (seed % 3), n_receivers=5,
                            width=60.0)
 
 
@@ -28,13 +28,27 @@ STREAM = [(0.00, "cased", 21), (0.01, "open", 31), (0.02, "cased", 22), (0.03, "
           (0.04, "cased", 23), (0.05, "open", 32), (0.06, "layered", 42)]
 
 
-def test_served_results_equal_solo_runs():
-    srv = serve.Server(max_slots=2, max_batch=2, chunk=64,
+def layered_class_threshold():
+    """A large_dofs that puts the toy layered requests (and only them) in
+    the large slot class."""
+    dofs = {kind: [make(kind, seed).build_setup().n_dofs for _, k, seed in STREAM if k == kind]
+            for kind in ("cased", "open", "layered")}
+    lo = max(dofs["cased"] + dofs["open"])
+    assert min(dofs["layered"]) > lo
+    return lo + 1
+
+
+@pytest.mark.parametrize("large_slots", [0, 1])
+def test_served_results_equal_solo_runs(large_slots):
+    large_dofs = layered_class_threshold() if large_slots else 0
+    srv = serve.Server(max_slots=2, max_batch=2, chunk=64, max_large_slots=large_slots, large_dofs=large_dofs,
                        stc=dict(slowness=np.linspace(100e-6, 1500e-6, 57), window=5e-5,
                                 times=np.linspace(0.0, 1.9e-4, 39)))
     reqs = srv.serve([(t, make(kind, seed)) for t, kind, seed in STREAM])
     assert [r.rid for r in reqs] == list(range(len(STREAM)))
     for r, (t, kind, seed) in zip(reqs, STREAM):
+        assert r.ok, r.error
+        assert r.large == (large_slots == 1 and kind == "layered")
         assert r.arrival == t
         assert r.admitted >= r.arrival and r.done > r.admitted
         alone = make(kind, seed).solver
@@ -64,5 +78,5 @@ def test_admission_respects_the_memory_budget():
     for (a0, d0), (a1, _) in zip(spans, spans[1:]):
         assert a1 >= d0 - 1e-9  # never two on the device at once
     small = serve.Server(hbm_budget=one // 4)
-    with pytest.raises(MemoryError):
-        small.serve([(0.0, make("cased", 21))])
+    refused, = small.serve([(0.0, make("cased", 21))])
+    assert not refused.ok and refused.error.startswith("refused") and not refused.traces
