@@ -193,11 +193,11 @@ def batched_leg(n_models, duration, plain_sample, batch_size=5, seed=2026):
     """Cased-hole dipole models (azimuthal order 1) drawn from the batched
     workload's distribution, solved end to end (host setup, power-iteration
     Δt, all steps, traces): all n_models on the GPU as batches of up to
-    batch_size models of one polynomial order and similar size, all of them
-    side by side on their own streams (the models' host setups built on
-    worker threads while the batches already created run); and the first
-    plain_sample of the same draw one after another on the plain path, whose
-    solves/s stands for sequential service of the whole set.  Every solve is
+    batch_size models of one polynomial order and similar step count, all
+    of them side by side on their own streams; and every k-th model of the
+    same draw (k = n_models // plain_sample, so that the sample spans the
+    draw's orders and sizes) one after another on the plain path.  The two
+    paths are compared per model-step (vs_plain) and per solve.  Every solve is
     queued at t = 0; a solve's latency is the time until its traces are
     ready."""
     import numpy as np
@@ -210,9 +210,10 @@ def batched_leg(n_models, duration, plain_sample, batch_size=5, seed=2026):
         return [cased_hole(rng, duration=duration) for _ in range(n_models)]
 
     n_plain = min(plain_sample, n_models)
+    stride = max(1, n_models // max(1, n_plain))
     t0 = time.perf_counter()
     plain_done, steps = [], 0
-    for sim in draw()[:n_plain]:
+    for sim in draw()[::stride][:n_plain]:
         s = sim.solver
         s.run()
         _ = s.traces
@@ -225,9 +226,8 @@ def batched_leg(n_models, duration, plain_sample, batch_size=5, seed=2026):
     sims = draw()
     orders = sorted({sim.discretization.order for sim in sims})
     batch_done = [0.0] * n_models
-    # insem_amd.solve_many: each order's models sorted by size and cut into
-    # batches of up to batch_size, all side by side on their own streams
-    # (host setups on worker threads while the batches already created run)
+    # insem_amd.solve_many: each order's models sorted by step count and cut
+    # into batches of up to batch_size, all side by side on their own streams
     n_batches = sum(-(-sum(1 for s in sims if s.discretization.order == o) // batch_size) for o in orders)
     t0 = time.perf_counter()
 
@@ -258,7 +258,7 @@ def batched_leg(n_models, duration, plain_sample, batch_size=5, seed=2026):
                            "plain_sequential_p50": pct(plain_done, 50), "plain_sequential_p99": pct(plain_done, 99)},
             "sample": f"{n_models} cased-hole dipole models (seed {seed}, orders {orders}, {duration * 1e3:g} ms each, "
                       f"{batch_steps} model-steps) as {n_batches} batches of up to {batch_size} models of one order "
-                      f"and similar size, side by side, end to end; plain: the first {n_plain} of them "
+                      f"and similar step count, side by side, end to end; plain: every {stride}th of them ({n_plain}) "
                       f"({steps} model-steps) one after another, end to end (latencies of that sequential queue)",
             "batches": n_batches}
 
@@ -289,6 +289,7 @@ def served_leg(n_requests, rate, sweep, slots=4, batch=8, seed=7):
         t0 = time.perf_counter()
         reqs = srv.serve(stream)
         out = serve.summary(reqs, time.perf_counter() - t0)
+        out["loop_thread_s"] = {k: round(v, 3) for k, v in srv.stats.items()}
         for q in reqs:
             if not q.ok:
                 progress(f"served leg: request {q.rid} ({kinds[q.rid]}) failed: {q.error}")

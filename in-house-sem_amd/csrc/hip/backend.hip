@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -84,6 +85,9 @@ struct BlockRT {
   ForceArgs args;
   int32_t* d_elem = nullptr;
   std::vector<int32_t> h_emodel;  // batch: model of each block element
+  // batch: elements a time-step launch covers at each level (the leading
+  // force batches that hold work at the level's steps; see insem_hip_ctx::floors)
+  std::vector<int> nb_at;
 };
 
 struct DomainRT {
@@ -149,6 +153,12 @@ struct insem_hip_ctx {
   // batch of independent models (0: a single model)
   int n_models = 0;
   ModelCoef* d_models = nullptr;
+  // The models' distinct step counts, largest first.  A chunk whose first
+  // step is s runs at level j, the place of the smallest count >= s: every
+  // force batch past the last one holding a model of count >= floors[j] is
+  // idle for the whole chunk, so the chunk's force launches leave them out.
+  std::vector<int32_t> floors;
+  int level = 0;  // of the chunk being launched or captured
   // power iteration in flight: runs started, host start vectors kept alive
   int lam_runs = 0;
   std::vector<std::vector<float>> lam_host;
@@ -920,6 +930,21 @@ StepClock clock_at(const insem_hip_ctx* c, int offset) {
   return k;
 }
 
+// A block's force arguments for a time step of the current chunk: on a
+// batch, only the leading elements that hold work at the chunk's level.
+ForceArgs level_args(const insem_hip_ctx* c, const BlockRT& br) {
+  ForceArgs a = br.args;
+  if (c->n_models && static_cast<size_t>(c->level) < br.nb_at.size()) a.nb = br.nb_at[c->level];
+  return a;
+}
+
+// The level of a chunk whose first step is s (0 for a single model).
+int level_of(const insem_hip_ctx* c, int s) {
+  int j = 0;
+  while (j + 1 < static_cast<int>(c->floors.size()) && c->floors[j + 1] >= s) ++j;
+  return j;
+}
+
 // One compute_acceleration + corrector (+ predictor) over both domains at
 // step *d_clock + offset.  Energy partials are written on energy steps only
 // (the kernels read the step on the device).
@@ -955,8 +980,9 @@ void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, 
     hipEvent_t e0 = nullptr;
     timed_begin(c, &e0);
     int grid = 0;
-    HIP_TRY(launch_forces_pair(c->ngll, pa->phys, pa->rect, pb->phys, pb->rect, pa->args, pb->args, p, s0, &grid,
-                               with_rec ? &c->rec : nullptr, with_rec ? c->rec.n_tasks : 0, 0.5 * c->dt));
+    HIP_TRY(launch_forces_pair(c->ngll, pa->phys, pa->rect, pb->phys, pb->rect, level_args(c, *pa),
+                               level_args(c, *pb), p, s0, &grid, with_rec ? &c->rec : nullptr,
+                               with_rec ? c->rec.n_tasks : 0, 0.5 * c->dt));
     timed_end(c, e0, true);
     rec_taken = true;
     row += grid;
@@ -975,8 +1001,8 @@ void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, 
       hipEvent_t e0 = nullptr;
       timed_begin(c, &e0);
       int grid = 0;
-      HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, true, br.args, p, s0, &grid, with_rec ? &c->rec : nullptr,
-                            with_rec ? c->rec.n_tasks : 0, 0.5 * c->dt));
+      HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, true, c->fuse ? level_args(c, br) : br.args, p, s0, &grid,
+                            with_rec ? &c->rec : nullptr, with_rec ? c->rec.n_tasks : 0, 0.5 * c->dt));
       timed_end(c, e0, true);
       rec_taken = rec_taken || with_rec;
       if (c->fuse) row += grid;
@@ -1012,6 +1038,7 @@ void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, 
 }
 
 void initialize(insem_hip_ctx* c) {
+  c->level = level_of(c, 0);
   step_body(c, 0, false, 0, 0, 0, false, true);
   c->initialized = 1;
 }
@@ -1036,10 +1063,11 @@ void chunk_launches(insem_hip_ctx* c, int k, bool first, bool last) {
 // Runs a chunk, replaying a captured hipGraph of it when graphs are on (the
 // step numbers come from the device clock, so one graph serves every chunk
 // of the same shape).  Timing runs launch directly, with events.
-hipGraphExec_t chunk_graph(insem_hip_ctx* c, int k, bool first, bool last) {
-  const int key = (k << 2) | (first ? 2 : 0) | (last ? 1 : 0);
+hipGraphExec_t chunk_graph(insem_hip_ctx* c, int k, bool first, bool last, int level) {
+  const int key = (level << 16) | (k << 2) | (first ? 2 : 0) | (last ? 1 : 0);
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
+    c->level = level;
     std::unique_lock<std::mutex> lock(g_capture_mu);
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipGraph_t g = nullptr;
@@ -1064,22 +1092,23 @@ hipGraphExec_t chunk_graph(insem_hip_ctx* c, int k, bool first, bool last) {
   return it->second;
 }
 
-void run_chunk(insem_hip_ctx* c, int k, bool first, bool last) {
+void run_chunk(insem_hip_ctx* c, int k, bool first, bool last, int level) {
   if (!c->use_graphs || c->timing) {
+    c->level = level;
     chunk_launches(c, k, first, last);
     return;
   }
-  HIP_TRY(hipGraphLaunch(chunk_graph(c, k, first, last), c->stream));
+  HIP_TRY(hipGraphLaunch(chunk_graph(c, k, first, last, level), c->stream));
 }
 
-// The chunks of an advance of n steps from the current step: (k, first, last)
-// for each, in order.
+// The chunks of an advance of n steps from the current step: (k, first,
+// last, level) for each, in order.
 template <class F>
 void for_each_chunk(const insem_hip_ctx* c, int n, F&& f) {
   const int m = std::max(0, std::min(n, c->n_steps - c->step));
   for (int s = 0; s < m; s += c->chunk) {
     const int k = std::min(c->chunk, m - s);
-    f(k, s == 0, s + k == m);
+    f(k, s == 0, s + k == m, level_of(c, c->step + s + 1));
   }
 }
 
@@ -1413,10 +1442,23 @@ int insem_hip_set_time_batch(insem_hip_ctx* c, const double* dt, const int32_t* 
     // read their coefficients uniformly; the model of each 256 points, for
     // the tails' uniform coefficients.
     const int epb = force_batch_elements(c->ngll);
+    c->floors.assign(n_steps, n_steps + c->n_models);
+    std::sort(c->floors.begin(), c->floors.end(), std::greater<int32_t>());
+    c->floors.erase(std::unique(c->floors.begin(), c->floors.end()), c->floors.end());
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+    c->graphs.clear();
     for (DomainRT& R : c->doms) {
       for (BlockRT& br : R.blocks) {
         std::vector<int32_t> last(static_cast<size_t>((br.nb + epb - 1) / epb), -1);
         for (int e = 0; e < br.nb; ++e) last[e / epb] = std::max(last[e / epb], n_steps[br.h_emodel[e]]);
+        // elements up to the end of the last force batch with work at each level
+        br.nb_at.assign(c->floors.size(), 0);
+        for (size_t j = 0; j < c->floors.size(); ++j)
+          for (int t = static_cast<int>(last.size()) - 1; t >= 0; --t)
+            if (last[t] >= c->floors[j]) {
+              br.nb_at[j] = std::min(br.nb, (t + 1) * epb);
+              break;
+            }
         if (!br.args.batch_last) br.args.batch_last = c->upload(last);
         else c->h2d(const_cast<int32_t*>(br.args.batch_last), last.data(), last.size() * sizeof(int32_t));
         if (!br.args.batch_model) {
@@ -1481,7 +1523,7 @@ int insem_hip_advance(insem_hip_ctx* c, int32_t n) {
     require_time(c);
     if (!c->initialized) initialize(c);
     const int m = std::max(0, std::min(n, c->n_steps - c->step));
-    for_each_chunk(c, m, [&](int k, bool first, bool last) { run_chunk(c, k, first, last); });
+    for_each_chunk(c, m, [&](int k, bool first, bool last, int level) { run_chunk(c, k, first, last, level); });
     c->step += m;
   });
 }
@@ -1490,7 +1532,7 @@ int insem_hip_prepare(insem_hip_ctx* c, int32_t n) {
   return guarded(c, [&] {
     require_time(c);
     if (!c->use_graphs) return;
-    for_each_chunk(c, n, [&](int k, bool first, bool last) { (void)chunk_graph(c, k, first, last); });
+    for_each_chunk(c, n, [&](int k, bool first, bool last, int level) { (void)chunk_graph(c, k, first, last, level); });
   });
 }
 

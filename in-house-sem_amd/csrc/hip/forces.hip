@@ -840,6 +840,7 @@ hipError_t launch_pair_one(const ForceArgs& a, const ForceArgs& b, const StepPar
   const int rec_end = up8(n_rec), na8 = up8(ba);
   const int grid = rec_end + na8 + bb;
   if (grid_out) *grid_out = grid;
+  if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL((k_forces_pair<NG, PA, RA, PB, RB>), dim3(grid), dim3(kWG), 0, s, a, b, na8, p,
                      rw.r ? *rw.r : ReceiverDev(), n_rec, rec_end, rw.half_dt);
   return hipGetLastError();

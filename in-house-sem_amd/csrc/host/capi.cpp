@@ -343,6 +343,17 @@ int insem_image_time(void* h, double* d, int32_t* i) {
   });
 }
 
+// The time step and step count a model gets from lambda_max (as its Solver
+// would): d = dt, stable_dt; *n = n_steps.
+int insem_image_model_time(void* h, double lambda, double* d, int32_t* n) {
+  return guard([&] {
+    const ModelTime m = model_time(static_cast<SetupImage*>(h)->time, lambda);
+    d[0] = m.dt;
+    d[1] = m.stable_dt;
+    *n = m.n_steps;
+  });
+}
+
 int insem_image_n_traces(void* h) {
   int n = 0;
   for (const auto& r : static_cast<SetupImage*>(h)->receivers) n += static_cast<int>(r.quantities.size());

@@ -125,6 +125,7 @@ def host():
         d("insem_image_start_vectors", c_int, c_void, P(ctypes.c_float))
         d("insem_image_amplitudes", c_int, c_void, c_double, c_int, P(ctypes.c_float))
         d("insem_image_time", c_int, c_void, P(c_double), P(c_int))
+        d("insem_image_model_time", c_int, c_void, c_double, P(c_double), P(c_int))
         d("insem_image_n_traces", c_int, c_void)
         d("insem_image_trace_info", c_int, c_void, c_int, c_char, c_int, c_char, c_int, P(c_int))
         d("insem_image_counts", c_int, c_void, P(c_i64))

@@ -287,6 +287,14 @@ class SetupImage:
         check(_lib.host().insem_image_start_vectors(self._h, _fptr(out)))
         return out
 
+    def time_step(self, lambda_max: float) -> Tuple[float, int]:
+        """(dt, n_steps) the model gets from lambda_max, as its Solver would
+        (reference: solver.cpp:131-144)."""
+        d = np.zeros(2)
+        n = ctypes.c_int32()
+        check(_lib.host().insem_image_model_time(self._h, float(lambda_max), _dptr(d), ctypes.byref(n)))
+        return float(d[0]), int(n.value)
+
     def amplitudes(self, dt: float, n_steps: int) -> np.ndarray:
         out = np.zeros((self.n_sources, n_steps + 1), dtype=np.float32)
         check(_lib.host().insem_image_amplitudes(self._h, dt, n_steps, _fptr(out)))

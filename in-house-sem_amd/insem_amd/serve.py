@@ -16,7 +16,9 @@ Requests (a Simulation each) arrive over time and go through four stages:
    small slot is free, the waiting small requests that can share a batch
    with the oldest one (same polynomial order, coordinate system, azimuthal
    order, media and record cadence; at most `max_batch`) are admitted
-   together as one BatchSolver, within the device-memory budget;
+   together as one BatchSolver, within the device-memory budget; among
+   those, the ones whose step counts are closest to the oldest one's come
+   first (a batch runs until its longest model ends);
 4. stepping: each slot is kept busy by queueing its next chunk of steps as
    soon as its stream is idle.  A request's traces are returned as soon as
    its own step count is reached (bitwise those of the request run alone;
@@ -38,6 +40,7 @@ from __future__ import annotations
 
 import ctypes
 import time
+from contextlib import contextmanager
 from collections import deque
 from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
@@ -147,6 +150,18 @@ class Server:
         self.max_power_jobs = max_power_jobs
         self.timeout = timeout
         self.workers = workers
+        self.stats: Dict[str, float] = {}
+        self.events: List[tuple] = []
+
+    @contextmanager
+    def _timed(self, name: str):
+        """Adds the time of the block to stats[name] (the loop thread's time
+        in each kind of device call)."""
+        t = time.perf_counter()
+        try:
+            yield
+        finally:
+            self.stats[name] = self.stats.get(name, 0.0) + time.perf_counter() - t
 
     # The device-facing steps, one method each (a test replaces them).
     def _setup(self, r: Request) -> SetupImage:
@@ -172,16 +187,21 @@ class Server:
         requests then fail)."""
         head = queue[0]
         limit = 1 if large else self.max_batch
+        # the oldest request, then those that can share its batch by how
+        # close their step counts are to its own (a batch runs as long as
+        # its longest model), oldest first among equals
+        same = [r for r in queue if r is not head and r.key == head.key]
+        same.sort(key=lambda r: abs(np.log(max(r.n_steps, 1) / max(head.n_steps, 1))))
         group, need = [], 0
-        for r in list(queue):
-            if r.key != head.key or len(group) == limit:
-                continue
-            if used + need + r.footprint > self.hbm_budget:
+        for r in [head] + same:
+            if len(group) == limit:
                 break
+            if used + need + r.footprint > self.hbm_budget:
+                if r is head:
+                    return None
+                continue
             group.append(r)
             need += r.footprint
-        if not group:
-            return None
         now = time.perf_counter() - t0
         for r in group:
             queue.remove(r)
@@ -190,13 +210,15 @@ class Server:
         for r in group:
             r.image = None  # the batch takes the images over
         try:
-            b = self._batch(images, [r.lam for r in group])
+            with self._timed("batch_setup_s"):
+                b = self._batch(images, [r.lam for r in group])
         except Exception as e:  # noqa: BLE001 - the batch's requests fail, the loop goes on
             for r in group:
                 self._fail(r, f"batch set-up failed: {e}", t0)
             return _FAILED
         for m, r in enumerate(group):
             r.dt, r.n_steps = b.dt(m), b.n_steps(m)
+        self.events.append((time.perf_counter() - t0, "admit", [r.rid for r in group], [r.n_steps for r in group]))
         return _Slot(b, group, need, large, list(range(len(group))))
 
     def _coherence(self, r: Request, step: float) -> None:
@@ -237,6 +259,9 @@ class Server:
         """Runs the stream (arrival times in s from now) to completion; returns
         the requests in arrival order with their timings and results."""
         reqs = [Request(i, sim, float(t)) for i, (t, sim) in enumerate(sorted(arrivals, key=lambda a: a[0]))]
+        self.stats = {"iterations": 0, "power_start_s": 0.0, "batch_setup_s": 0.0, "advance_s": 0.0,
+                      "deliver_s": 0.0}
+        self.events = []  # (s since start, "admit", request ids, their step counts)
         pending = deque(reqs)
         building: deque = deque()   # (request, future of its setup) in arrival order
         timing: deque = deque()     # requests waiting for a power-iteration slot
@@ -275,7 +300,8 @@ class Server:
                 while timing and len(power) < self.max_power_jobs and used + timing[0].footprint <= self.hbm_budget:
                     r = timing.popleft()
                     try:
-                        power.append((r, self._power_job(r.image)))
+                        with self._timed("power_start_s"):
+                            power.append((r, self._power_job(r.image)))
                         used += r.footprint
                     except Exception as e:  # noqa: BLE001
                         self._fail(r, f"power iteration failed: {e}", t0)
@@ -288,6 +314,7 @@ class Server:
                     used -= r.footprint
                     try:
                         r.lam = job.lambda_max()
+                        r.dt, r.n_steps = r.image.time_step(r.lam)
                     except Exception as e:  # noqa: BLE001
                         self._fail(r, f"power iteration failed: {e}", t0)
                         r.image = None
@@ -322,7 +349,8 @@ class Server:
                     step = b.step
                     for m in [m for m in slot.open if slot.requests[m].n_steps <= step]:
                         slot.open.remove(m)
-                        j = self._deliver(slot, m, pool, t0)
+                        with self._timed("deliver_s"):
+                            j = self._deliver(slot, m, pool, t0)
                         if j is not None:
                             jobs.append(j)
                     for m in [m for m in slot.open if expired(slot.requests[m], now)]:
@@ -333,11 +361,13 @@ class Server:
                         slots.remove(slot)
                         used -= slot.footprint
                     else:
-                        b.advance(self.chunk)
+                        with self._timed("advance_s"):
+                            b.advance(self.chunk)
                         busy = True
                 for j in [j for j in jobs if j.done()]:
                     j.result()
                     jobs.remove(j)
+                self.stats["iterations"] += 1
                 if not busy:
                     idle = not (building or timing or power or queues[False] or queues[True] or slots or jobs)
                     if idle and pending:

@@ -47,6 +47,21 @@ def test_cased_hole_batch_equals_alone(order):
         assert_same_traces(batch.traces(m), alone.traces)
 
 
+def test_batch_results_do_not_depend_on_model_order():
+    # the batch steps its models longest first whatever order they come in;
+    # each model keeps its place in the caller's list
+    seeds = [11, 12, 13, 14]
+    fwd = insem.BatchSolver([small_cased(s) for s in seeds])
+    rev = insem.BatchSolver([small_cased(s) for s in reversed(seeds)])
+    steps = [fwd.n_steps(m) for m in range(4)]
+    assert len(set(steps)) > 1, steps  # then at least one of the two orders is not longest first
+    fwd.run()
+    rev.run()
+    for m in range(4):
+        assert fwd.n_steps(m) == rev.n_steps(3 - m) and fwd.dt(m) == rev.dt(3 - m)
+        assert_same_traces(fwd.traces(m), rev.traces(3 - m))
+
+
 def test_coupled_cartesian_batch_equals_alone_and_oracle():
     def make(k):
         return layered_coupled(nx=24 + 8 * k, nz_fluid=3, nz_solid=(5 + k, 6), duration=0.02 + 0.01 * k,

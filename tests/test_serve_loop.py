@@ -27,6 +27,9 @@ class FakeImage:
         self.time = SimpleNamespace(record_every=1)
         self.n_dofs = 3 * spec["points"]
 
+    def time_step(self, lam):
+        return 2.0 / np.sqrt(lam), self.spec["steps"]
+
 
 def solo_traces(spec):
     """The traces a model gives alone: a function of its own spec only."""
@@ -250,3 +253,17 @@ def test_timeout_stops_a_request_and_its_batch():
     assert reqs[1].error.startswith("timed out")
     assert reqs[1].done - reqs[1].arrival < 1.0
     assert ("closed", [1]) in srv.log  # its batch was stopped
+
+
+def test_batches_group_similar_step_counts():
+    # request 9 holds the only slot while the others get their time steps;
+    # then the oldest waiting request (0, 100 steps) is batched with the two
+    # whose step counts are closest to its own
+    srv = FakeServer(max_slots=1, max_batch=3, chunk=50)
+    specs = [spec(9, steps=400, power_polls=0), spec(0, steps=100), spec(1, steps=1000), spec(2, steps=120),
+             spec(3, steps=5000), spec(4, steps=90)]
+    reqs = srv.serve([(0.0, s) for s in specs])
+    for r in reqs:
+        check_solo(r)
+    batches = [e[1] for e in srv.log if e[0] == "batch"]
+    assert batches[0] == [9] and sorted(batches[1]) == [0, 2, 4] and sorted(batches[2]) == [1, 3]
