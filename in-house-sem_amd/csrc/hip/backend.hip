@@ -1379,6 +1379,9 @@ int create_impl(const insem_hip_setup* s0, const insem_hip_batch* batch, insem_h
 void apply_time(insem_hip_ctx* c, double dt, int32_t n_steps, const float* amplitudes) {
   c->dt = dt;
   c->n_steps = n_steps;
+  // captured chunks hold the previous amplitude and trace buffers
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  c->graphs.clear();
   const size_t na = static_cast<size_t>(c->n_sources) * (static_cast<size_t>(n_steps) + 1);
   c->amps = c->upload(amplitudes ? amplitudes : nullptr, amplitudes ? na : 0);
   for (DomainRT& R : c->doms) {

@@ -64,6 +64,31 @@ const char* Source::type_name() const {
 
 const char* medium_name(Medium m) { return m == Medium::Acoustic ? "acoustic" : "elastic"; }
 
+const char* boundary_name(Boundary b) {
+  return b == Boundary::Free ? "free" : b == Boundary::Rigid ? "rigid" : "absorbing";
+}
+
+Boundary boundary_from_name(const std::string& s) {
+  for (Boundary b : {Boundary::Free, Boundary::Rigid, Boundary::Absorbing})
+    if (s == boundary_name(b)) return b;
+  throw Error("boundary type '" + s + "' is none of 'free', 'rigid', 'absorbing'");
+}
+
+Medium medium_from_name(const std::string& s) {
+  if (s == "acoustic") return Medium::Acoustic;
+  if (s == "elastic") return Medium::Elastic;
+  throw Error("medium '" + s + "' is neither 'acoustic' nor 'elastic'");
+}
+
+const char* stf_name(int type) {
+  static const char* names[] = {"ricker", "gaussian", "gaussian_derivative", "samples"};
+  return type >= 0 && type < 4 ? names[type] : "samples";
+}
+
+// The version this backend reports as insem's (the reference project's
+// version, CMakeLists.txt:3), so that files and front ends agree on it.
+const char* insem_version() { return "0.1.0"; }
+
 // Components of a recorded quantity (description.hpp:225-244): elastic
 // displacement, velocity and acceleration have 2 (m = 0) or 3 (m = 1:
 // U_r, U_theta, U_z); acoustic pressure 1 and particle velocity 2.
@@ -109,6 +134,7 @@ void Model::set_material(int region, const Material& m) {
   if (disc_.adaptive()) {  // wavelength refinement depends on the materials
     forest_.reset();
     geom_.reset();
+    dofs_.reset();
   }
 }
 void Model::set_boundary(int tag, Boundary b) { boundaries_[tag] = b; }
@@ -194,6 +220,7 @@ void Model::setup() {
   const int ngll = disc_.order + 1;
   gll_ = gll_rule(ngll);
   forest_ = build_forest();
+  dofs_.reset();
   geom_ = compute_geometry(*forest_, gll_->points, glj01_rule(ngll).points);
 }
 
@@ -205,6 +232,15 @@ const Geometry& Model::geometry() const {
   INSEM_REQUIRE(is_setup(), "the model is not set up yet (call setup first)");
   return *geom_;
 }
+const DofMap& Model::dofs() const {
+  if (!dofs_) {
+    std::vector<int32_t> all(static_cast<std::size_t>(forest().n_leaves()));
+    for (int32_t l = 0; l < forest().n_leaves(); ++l) all[l] = l;
+    dofs_ = DofMap::build(forest(), disc_.order + 1, all);
+  }
+  return *dofs_;
+}
+
 const Rule& Model::gll() const {
   INSEM_REQUIRE(is_setup(), "the model is not set up yet (call setup first)");
   return *gll_;

@@ -68,6 +68,11 @@ struct TimeConfig {
   double duration = 0.0;
   int record_every = 1;
   int energy_every = 0;
+  // Snapshots of whole fields every snapshot_every steps (0: none), of the
+  // listed quantities that exist in each medium (displacement, velocity,
+  // acceleration in solids; pressure in fluids).
+  int snapshot_every = 0;
+  std::vector<std::string> snapshot_quantities{"velocity", "pressure"};
 };
 
 struct Region {
@@ -111,6 +116,9 @@ struct SetupImage {
     std::vector<float> absorbing_coef;
     std::vector<int32_t> dirichlet_points, axis_points, axis_components;
     std::vector<double> coords;  // [2][n_global]
+    // coordinates of every element point [e * npe + k] (models that take
+    // snapshots only)
+    std::vector<double> point_x, point_z;
   };
   struct SourceData {
     int domain = 0;
@@ -174,7 +182,12 @@ class Model {
   Model(std::shared_ptr<const MacroMesh> mesh, Discretization disc);
 
   const MacroMesh& mesh() const { return *mesh_; }
+  std::shared_ptr<const MacroMesh> mesh_ptr() const { return mesh_; }
   const Discretization& discretization() const { return disc_; }
+  const std::map<int, Material>& materials() const { return materials_; }
+  const std::map<int, Boundary>& boundaries() const { return boundaries_; }
+  const std::vector<Source>& sources() const { return sources_; }
+  const std::vector<Receiver>& receivers() const { return receivers_; }
   void set_material(int region, const Material& m);
   void set_boundary(int tag, Boundary b);
   void add_source(const Source& s);
@@ -189,6 +202,8 @@ class Model {
   const Forest& forest() const;
   const Geometry& geometry() const;
   const Rule& gll() const;
+  // The point map of all leaves (both media), built on first use.
+  const DofMap& dofs() const;
   std::map<std::string, int32_t> kind_counts() const;
 
   // The time-loop state (reference Solver constructor, minus the time step).
@@ -206,6 +221,16 @@ class Model {
   std::optional<Rule> gll_;
   std::optional<Forest> forest_;
   std::optional<Geometry> geom_;
+  mutable std::optional<DofMap> dofs_;
 };
+
+// Names of the description's enumerations as the reference's files and
+// Python interface spell them (cpp/src/sim/description.cpp:17-59,
+// cpp/src/mesh/macro_mesh.cpp:54-70); the parsers throw on other names.
+const char* boundary_name(Boundary b);
+Boundary boundary_from_name(const std::string& s);
+Medium medium_from_name(const std::string& s);
+const char* stf_name(int type);
+const char* insem_version();
 
 }  // namespace insem_amd

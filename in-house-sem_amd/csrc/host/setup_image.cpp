@@ -77,6 +77,12 @@ class ImageBuilder {
     INSEM_REQUIRE(ngll_ >= 2 && ngll_ <= 9, "the kernels take polynomial orders 1 to 8, not " << ngll_ - 1);
     INSEM_REQUIRE(time.record_every >= 1 && time.energy_every >= 0,
                   "record_every must be at least 1 and energy_every at least 0");
+    INSEM_REQUIRE(time.snapshot_every >= 0, "snapshot_every must be at least 0, not " << time.snapshot_every);
+    if (time.snapshot_every > 0)
+      for (const std::string& q : time.snapshot_quantities)
+        INSEM_REQUIRE(q == "displacement" || q == "velocity" || q == "acceleration" || q == "pressure",
+                      "snapshots record displacement, velocity, acceleration or pressure, not '" << q << "'");
+    snapshots_ = time.snapshot_every > 0;
     for (int32_t c = 0; c < M_.n_cells(); ++c) {
       const auto it = materials_.find(M_.material(c));
       INSEM_REQUIRE(it != materials_.end(), "region " << M_.material(c) << " has no material");
@@ -98,6 +104,7 @@ class ImageBuilder {
   }
 
  private:
+  bool snapshots_ = false;  // keep the element points' coordinates
   const Material& material(int32_t leaf) const { return materials_.at(M_.material(F_.leaf(leaf).tree)); }
   std::size_t point_of(int32_t leaf, int i, int j) const {
     return static_cast<std::size_t>(leaf) * npe_ + static_cast<std::size_t>(j) * ngll_ + i;
@@ -193,6 +200,16 @@ class ImageBuilder {
       d.coords[g] = G_.x[p];
       d.coords[ng + g] = G_.z[p];
       if (!d.axis_components.empty() && G_.x[p] <= 0.0) d.axis_points.push_back(g);
+    }
+    if (snapshots_) {
+      d.point_x.resize(ne * npe_);
+      d.point_z.resize(ne * npe_);
+      for (std::size_t e = 0; e < ne; ++e)
+        for (int k = 0; k < npe_; ++k) {
+          const std::size_t p = static_cast<std::size_t>(d.leaves[e]) * npe_ + k;
+          d.point_x[e * npe_ + k] = G_.x[p];
+          d.point_z[e * npe_ + k] = G_.z[p];
+        }
     }
     img_->domains.push_back(std::move(d));
   }

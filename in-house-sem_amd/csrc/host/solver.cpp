@@ -1,5 +1,6 @@
 #include "solver.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 #include "error.hpp"
@@ -46,6 +47,14 @@ Solver::Solver(std::unique_ptr<SetupImage> image) : img_(std::move(image)) {
     n_steps_ = m.n_steps;
     const std::vector<float> amps = img_->source_amplitudes(dt_, n_steps_);
     check_hip(insem_hip_set_time(ctx_, dt_, n_steps_, amps.data()));
+    if (img_->time.snapshot_every > 0)
+      for (const std::string& q : img_->time.snapshot_quantities)
+        for (const auto& d : img_->domains) {
+          const bool has = d.medium == Medium::Elastic
+                               ? (q == "displacement" || q == "velocity" || q == "acceleration")
+                               : q == "pressure";
+          if (has) snapshots_.push_back({d.medium, q, d.ncomp, {}});
+        }
   } catch (...) {
     insem_hip_destroy(ctx_);
     ctx_ = nullptr;
@@ -63,7 +72,106 @@ int Solver::step() const {
   return s;
 }
 
-void Solver::advance(int n) { check_hip(insem_hip_advance(ctx_, n)); }
+// With snapshots the device runs to each snapshot step in turn (an advance
+// call that ends on a step leaves the fields as the reference's step does;
+// splitting a run into calls changes no value).
+void Solver::advance(int n) {
+  const int every = img_->time.snapshot_every;
+  if (every <= 0) {
+    check_hip(insem_hip_advance(ctx_, n));
+    return;
+  }
+  if (!started_) {
+    check_hip(insem_hip_advance(ctx_, 0));  // initialisation only
+    record_snapshot();
+    started_ = true;
+  }
+  int s = step();
+  const int target = std::min(n_steps_, s + std::max(n, 0));
+  while (s < target) {
+    const int next = std::min(target, (s / every + 1) * every);
+    check_hip(insem_hip_advance(ctx_, next - s));
+    s = next;
+    if (s % every == 0) record_snapshot();
+  }
+}
+
+const SetupImage::Domain& Solver::domain_of(Medium m) const {
+  const int di = img_->domain_index(m);
+  INSEM_REQUIRE(di >= 0, "the model has no " << medium_name(m) << " region");
+  return img_->domains[di];
+}
+
+// One snapshot: the device field gathered to the element points as
+// DofMap::gather does it (dof_map.hpp:94-120: constrained slots sum
+// float(weight) * value in row order, in float).
+void Solver::record_snapshot() {
+  snapshot_times_.push_back(time());
+  for (Snapshot& sn : snapshots_) {
+    const int di = img_->domain_index(sn.medium);
+    const auto& d = img_->domains[di];
+    const auto& M = d.dofs;
+    const int which = sn.quantity == "velocity" ? INSEM_FIELD_V : sn.quantity == "acceleration" ? INSEM_FIELD_A
+                                                                                                   : INSEM_FIELD_U;
+    const std::size_t ng = static_cast<std::size_t>(M.n_global), npe = static_cast<std::size_t>(M.npe);
+    const std::size_t n = static_cast<std::size_t>(M.n_elements) * npe;
+    std::vector<float> g(static_cast<std::size_t>(d.ncomp) * ng);
+    check_hip(insem_hip_get_field(ctx_, di, which, g.data()));
+    const std::size_t base = sn.data.size();
+    sn.data.resize(base + static_cast<std::size_t>(d.ncomp) * n);
+    float* out = &sn.data[base];
+    for (int c = 0; c < d.ncomp; ++c) {
+      const float* gc = &g[c * ng];
+      float* oc = out + c * n;
+      for (std::size_t s = 0; s < n; ++s) {
+        const int32_t m = M.l2g[s];
+        if (m >= 0) {
+          oc[s] = gc[m];
+        } else {
+          const int32_t r = -1 - m;
+          float acc = 0.0f;
+          for (int32_t q = M.row_offsets[r]; q < M.row_offsets[r + 1]; ++q)
+            acc += static_cast<float>(M.row_weights[q]) * gc[M.row_dofs[q]];
+          oc[s] = acc;
+        }
+      }
+    }
+    if (sn.medium == Medium::Elastic && d.azimuthal_order == 1) {
+      // (U+, U-) -> (U_r, U_theta) = ((U+ + U-), (U+ - U-)) / sqrt 2, in double
+      const double h = 0.70710678118654752440;
+      for (std::size_t s = 0; s < n; ++s) {
+        const double up = out[s], um = out[n + s];
+        out[s] = static_cast<float>(h * (up + um));
+        out[n + s] = static_cast<float>(h * (up - um));
+      }
+    }
+  }
+}
+
+std::vector<double> Solver::snapshot_points(Medium m) const {
+  const auto& d = domain_of(m);
+  const std::size_t n = d.point_x.size();
+  INSEM_REQUIRE(n > 0 || d.dofs.n_elements == 0, "the model takes no snapshots (snapshot_every is 0)");
+  std::vector<double> out(2 * n);
+  std::copy(d.point_x.begin(), d.point_x.end(), out.begin());
+  std::copy(d.point_z.begin(), d.point_z.end(), out.begin() + static_cast<std::ptrdiff_t>(n));
+  return out;
+}
+
+// Every element's (ngll - 1)^2 sub-quads over its points, counter-clockwise.
+std::vector<int32_t> Solver::snapshot_cells(Medium m) const {
+  const auto& d = domain_of(m);
+  const int ng = d.dofs.ngll, npe = d.dofs.npe;
+  std::vector<int32_t> out;
+  out.reserve(static_cast<std::size_t>(d.dofs.n_elements) * (ng - 1) * (ng - 1) * 4);
+  for (int32_t e = 0; e < d.dofs.n_elements; ++e)
+    for (int j = 0; j + 1 < ng; ++j)
+      for (int i = 0; i + 1 < ng; ++i) {
+        const int32_t p = e * npe + j * ng + i;
+        out.insert(out.end(), {p, p + 1, p + ng + 1, p + ng});
+      }
+  return out;
+}
 
 void Solver::set_state(Medium m, const std::vector<double>& u, const std::vector<double>& v) {
   const int di = img_->domain_index(m);

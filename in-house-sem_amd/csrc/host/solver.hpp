@@ -49,13 +49,34 @@ class Solver {
   std::vector<double> sample_times() const;
   std::vector<std::array<double, 3>> energy() const;
 
+  // Snapshots (solver.cpp:813-876): after initialisation and after every
+  // snapshot_every-th step, each listed quantity of each medium that has it,
+  // on the element points ([n_snapshots][ncomp][n_elements * npe], elements
+  // in the domain's leaf order; m = 1 solids as U_r, U_theta, U_z).
+  struct Snapshot {
+    Medium medium = Medium::Elastic;
+    std::string quantity;
+    int ncomp = 0;
+    std::vector<float> data;
+  };
+  const std::vector<Snapshot>& snapshots() const { return snapshots_; }
+  const std::vector<double>& snapshot_times() const { return snapshot_times_; }
+  std::vector<double> snapshot_points(Medium m) const;   // [2][n_points]
+  std::vector<int32_t> snapshot_cells(Medium m) const;   // [n_cells][4] sub-quads
+
   insem_hip_ctx* context() const { return ctx_; }
 
  private:
   std::unique_ptr<SetupImage> img_;
+  void record_snapshot();
+  const SetupImage::Domain& domain_of(Medium m) const;
+
   insem_hip_ctx* ctx_ = nullptr;
   double dt_ = 0.0, stable_dt_ = 0.0;
   int n_steps_ = 0;
+  std::vector<Snapshot> snapshots_;
+  std::vector<double> snapshot_times_;
+  bool started_ = false;  // the step-0 snapshot is taken
 };
 
 // Throws insem_amd::Error with the backend's message when rc != 0.
