@@ -43,9 +43,11 @@ sys.path.insert(0, os.path.join(ROOT, "in-house-sem_amd"))
 HBM_PEAK_GBS = 8000.0
 OPEN_HOLE_WORKLOAD = ("axisymmetric open-hole monopole, order 4 with GLJ on the axis, 20 kHz AMR, formation radius 4 m, "
                       "13 receivers")
-# HBM traffic of the dominant force launch, measured with rocprofv3 --pmc
-# (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of known size).
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_forces_pair_layered_800x400.txt")
+# HBM traffic of the dominant force launch of each workload, measured with
+# rocprofv3 --pmc (the L2's memory-side read and write requests resolved by
+# request size), from the committed counter summaries.
+TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f) for f in
+                 ("pmc_forces_pair_layered_800x400.txt", "pmc_open_hole_monopole.txt")]
 
 
 def progress(msg):
@@ -92,18 +94,20 @@ def algorithmic_bytes(image, hip, ctx, check):
 
 
 def measured_traffic(workload):
-    """HBM bytes per launch from the committed counter summary, when it was
-    measured on this workload ("traffic_bytes_per_launch: N" line)."""
-    try:
-        with open(TRAFFIC_FILE) as fh:
-            text = fh.read()
-    except OSError:
-        return None
-    w = re.search(r"^workload: (.*)$", text, re.M)
-    t = re.search(r"^traffic_bytes_per_launch: ([0-9.eE+]+)$", text, re.M)
-    if not w or not t or w.group(1).strip() != workload:
-        return None
-    return float(t.group(1))
+    """HBM bytes per launch of the dominant force launch from the committed
+    counter summary measured on this workload ("workload:" and
+    "traffic_bytes_per_launch:" lines), or None."""
+    for path in TRAFFIC_FILES:
+        try:
+            with open(path) as fh:
+                text = fh.read()
+        except OSError:
+            continue
+        w = re.search(r"^workload: (.*)$", text, re.M)
+        t = re.search(r"^traffic_bytes_per_launch: ([0-9.eE+]+)$", text, re.M)
+        if w and t and w.group(1).strip() == workload:
+            return float(t.group(1))
+    return None
 
 
 def cpu_baseline(sim, dt, max_seconds=20.0):
@@ -184,6 +188,7 @@ def plain_leg(sim, workload, steps, warmup):
            "force_ms_per_step": force_ms, "force_launches_per_step": m["force_n"] / m["timing_steps"],
            "force_algorithmic_bytes": force_bytes,
            "force_frac": force_bytes / (force_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+           "force_traffic_bytes": measured_traffic(workload),
            "tail_ms_per_step": m["tail_ms"] / m["timing_steps"]}
     solver.close()
     return out
