@@ -343,6 +343,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-open-hole", action="store_true", help="skip the open-hole monopole leg")
     ap.add_argument("--calibrate-copy", action="store_true", help="run only a 1 GiB device copy (counter scale)")
+    ap.add_argument("--only-cased-hole", type=int, default=0, metavar="ORDER",
+                    help="run only one cased-hole dipole model of this order (4-6) from the batched workload's "
+                         "distribution, alone on the plain path, and print its figures (for profiling its kernels)")
     ap.add_argument("--only-open-hole", action="store_true",
                     help="run only the open-hole monopole leg and print its figures (for profiling it)")
     ap.add_argument("--batch-models", type=int, default=128, help="models of the batched leg (0: skip it)")
@@ -364,6 +367,16 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.only_cased_hole:
+        import numpy as np
+
+        from insem_amd.configs import cased_hole
+
+        sim = cased_hole(np.random.default_rng(2026), order=args.only_cased_hole)
+        workload = (f"axisymmetric cased-hole dipole (azimuthal order 1), order {args.only_cased_hole}, "
+                    f"batched workload draw seed 2026, 8 receivers, alone on the plain path")
+        print(json.dumps(plain_leg(sim, workload, args.steps, args.warmup)), flush=True)
         return
     if args.only_open_hole:
         from insem_amd.configs import open_hole_monopole

@@ -115,6 +115,9 @@ struct DomainRT {
   int cpl_role = 0;
 };
 
+// The longest captured chunk (graph keys hold k in 14 bits).
+constexpr int kMaxChunk = 1024;
+
 struct TraceCol {
   int receiver, quantity, raw_ncomp, col, m1_elastic;
 };
@@ -146,10 +149,12 @@ struct insem_hip_ctx {
   // step clock and captured step chunks
   int32_t* d_clock = nullptr;  // steps done, on the device
   int chunk = 32;
+  int chunk_set = 0;  // steps per chunk fixed by the caller (else the tuned table's, or 32)
   int use_graphs = 1;
   int fuse = 1;  // owned points updated inside the force launches
   int pair = 1;  // a step's elastic and acoustic force work in one launch
   int local = 1;  // points whose slots share one batch are updated in the force launch
+  int force_variant = -1;  // autotuner variant of the paired force launch (-1: the tuned table's)
   // batch of independent models (0: a single model)
   int n_models = 0;
   ModelCoef* d_models = nullptr;
@@ -982,7 +987,7 @@ void step_body(insem_hip_ctx* c, int offset, bool implicit, int corr, int pred, 
     int grid = 0;
     HIP_TRY(launch_forces_pair(c->ngll, pa->phys, pa->rect, pb->phys, pb->rect, level_args(c, *pa),
                                level_args(c, *pb), p, s0, &grid, with_rec ? &c->rec : nullptr,
-                               with_rec ? c->rec.n_tasks : 0, 0.5 * c->dt));
+                               with_rec ? c->rec.n_tasks : 0, 0.5 * c->dt, c->force_variant));
     timed_end(c, e0, true);
     rec_taken = true;
     row += grid;
@@ -1359,6 +1364,11 @@ int create_impl(const insem_hip_setup* s0, const insem_hip_batch* batch, insem_h
     c->d_clock = c->alloc<int32_t>(1);
     c->zero(c->d_clock, sizeof(int32_t));
     if (const char* g = std::getenv("INSEM_HIP_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
+    if (const char* v = std::getenv("INSEM_HIP_FORCE_VARIANT")) c->force_variant = std::atoi(v);
+    if (const char* k = std::getenv("INSEM_HIP_CHUNK")) {
+      const int n = std::atoi(k);
+      if (n >= 1 && n <= kMaxChunk) c->chunk = n, c->chunk_set = 1;
+    }
     if (const char* f = std::getenv("INSEM_HIP_FUSE")) c->fuse = std::atoi(f) != 0;
     if (const char* f = std::getenv("INSEM_HIP_PAIR")) c->pair = std::atoi(f) != 0;
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1376,9 +1386,26 @@ int create_impl(const insem_hip_setup* s0, const insem_hip_batch* batch, insem_h
 }
 
 // dt, step count and amplitude table; trace and energy buffers sized for it.
+// The paired launch's first blocks (the elastic and the acoustic domain's
+// first), or nulls when the model has none.
+std::pair<const BlockRT*, const BlockRT*> paired_blocks(const insem_hip_ctx* c) {
+  if (c->solid < 0 || c->fluid < 0 || !c->pair || !c->fuse || c->doms[c->solid].blocks.empty() ||
+      c->doms[c->fluid].blocks.empty())
+    return {nullptr, nullptr};
+  const BlockRT& a = c->doms[c->solid].blocks[0];
+  const BlockRT& b = c->doms[c->fluid].blocks[0];
+  if (!forces_pair_supported(a.phys, a.rect, b.phys, b.rect)) return {nullptr, nullptr};
+  return {&a, &b};
+}
+
 void apply_time(insem_hip_ctx* c, double dt, int32_t n_steps, const float* amplitudes) {
   c->dt = dt;
   c->n_steps = n_steps;
+  if (!c->chunk_set) {
+    const auto [a, b] = paired_blocks(c);
+    const int k = a ? tuned_chunk_steps(c->ngll, a->phys, a->rect, b->phys, b->rect) : 0;
+    c->chunk = k >= 1 && k <= kMaxChunk ? k : 32;
+  }
   // captured chunks hold the previous amplitude and trace buffers
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
   c->graphs.clear();
@@ -1529,6 +1556,56 @@ int insem_hip_advance(insem_hip_ctx* c, int32_t n) {
     for_each_chunk(c, m, [&](int k, bool first, bool last, int level) { run_chunk(c, k, first, last, level); });
     c->step += m;
   });
+}
+
+int insem_hip_set_force_variant(insem_hip_ctx* c, int32_t variant) {
+  if (!c) return set_error(INSEM_HIP_EINVAL, "null context");
+  if (variant < -1 || variant >= kForceVariants)
+    return set_error(INSEM_HIP_EINVAL, "force variant %d is outside -1 .. %d", variant, kForceVariants - 1);
+  return guarded(c, [&] {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->force_variant = variant;
+    // captured chunks hold the previous variant's kernels
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+  });
+}
+
+int insem_hip_force_variant(insem_hip_ctx* c, int32_t* variant, int32_t* paired) {
+  if (!c || !variant || !paired) return set_error(INSEM_HIP_EINVAL, "null argument");
+  *variant = 0;
+  *paired = 0;
+  const auto [a, b] = paired_blocks(c);
+  if (a) {
+    *paired = 1;
+    *variant = c->force_variant >= 0 ? c->force_variant : tuned_force_variant(c->ngll, a->phys, a->rect, b->phys, b->rect);
+    if (c->ngll < 5 || c->ngll > 7) *variant = 0;
+  }
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_set_chunk(insem_hip_ctx* c, int32_t steps) {
+  if (!c) return set_error(INSEM_HIP_EINVAL, "null context");
+  if (steps < 0 || steps > kMaxChunk) return set_error(INSEM_HIP_EINVAL, "chunk of %d steps is outside 0 .. %d", steps, kMaxChunk);
+  return guarded(c, [&] {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->chunk_set = steps > 0;
+    if (steps > 0) {
+      c->chunk = steps;
+    } else {
+      const auto [a, b] = paired_blocks(c);
+      const int k = a ? tuned_chunk_steps(c->ngll, a->phys, a->rect, b->phys, b->rect) : 0;
+      c->chunk = k >= 1 && k <= kMaxChunk ? k : 32;
+    }
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+  });
+}
+
+int insem_hip_chunk(insem_hip_ctx* c, int32_t* steps) {
+  if (!c || !steps) return set_error(INSEM_HIP_EINVAL, "null argument");
+  *steps = c->chunk;
+  return INSEM_HIP_OK;
 }
 
 int insem_hip_prepare(insem_hip_ctx* c, int32_t n) {

@@ -739,9 +739,11 @@ constexpr int pair_waves(int ng, int pa, int pb) {
   return force_waves(ng, pa) < force_waves(ng, pb) ? force_waves(ng, pa) : force_waves(ng, pb);
 }
 
-template <int NG, int PA, bool RA, int PB, bool RB>
-__global__ __launch_bounds__(kWG, pair_waves(NG, PA, PB)) void k_forces_pair(ForceArgs A, ForceArgs B, int na8, StepParams P, ReceiverDev R, int n_rec, int rec_end,
-              double half_dt) {
+// WAVES: the waves per SIMD the kernel's registers are budgeted for (the
+// occupancy variants of the autotuner; pair_waves by default).
+template <int NG, int PA, bool RA, int PB, bool RB, int WAVES = pair_waves(NG, PA, PB)>
+__global__ __launch_bounds__(kWG, WAVES) void k_forces_pair(ForceArgs A, ForceArgs B, int na8, StepParams P,
+                                                          ReceiverDev R, int n_rec, int rec_end, double half_dt) {
   constexpr size_t la = ForceLds<NG, PA, RA>::bytes, lb = ForceLds<NG, PB, RB>::bytes;
   __shared__ __align__(16) unsigned char smem[la > lb ? la : lb];
   __shared__ double red[kWG];
@@ -830,9 +832,18 @@ hipError_t launch_gather_mode(int ngll, int phys, bool rect, const ForceArgs& a,
 
 namespace {
 
+// The LDS a work-group of the paired kernel declares.
 template <int NG, int PA, bool RA, int PB, bool RB>
+constexpr size_t pair_lds() {
+  constexpr size_t la = ForceLds<NG, PA, RA>::bytes, lb = ForceLds<NG, PB, RB>::bytes;
+  return (la > lb ? la : lb) + kWG * sizeof(double);
+}
+
+// wgs_per_cu > 0: the launch asks for enough dynamic LDS (unused) that at
+// most that many work-groups, i.e. waves per SIMD, fit on a CU's 160 KiB.
+template <int NG, int PA, bool RA, int PB, bool RB, int WAVES = pair_waves(NG, PA, PB)>
 hipError_t launch_pair_one(const ForceArgs& a, const ForceArgs& b, const StepParams& p, const RecWork& rw,
-                           hipStream_t s, int* grid_out) {
+                           hipStream_t s, int* grid_out, int wgs_per_cu = 0) {
   constexpr int EPB = kWG / (NG * NG);
   auto up8 = [](int n) { return (n + 7) / 8 * 8; };
   const int ba = (a.nb + EPB - 1) / EPB, bb = (b.nb + EPB - 1) / EPB;
@@ -841,7 +852,13 @@ hipError_t launch_pair_one(const ForceArgs& a, const ForceArgs& b, const StepPar
   const int grid = rec_end + na8 + bb;
   if (grid_out) *grid_out = grid;
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_forces_pair<NG, PA, RA, PB, RB>), dim3(grid), dim3(kWG), 0, s, a, b, na8, p,
+  size_t dyn = 0;
+  if (wgs_per_cu > 0) {
+    constexpr size_t kLdsPerCu = 160 * 1024, st = pair_lds<NG, PA, RA, PB, RB>();
+    const size_t want = (kLdsPerCu / (wgs_per_cu + 1) + 1024) / 512 * 512;  // > total / (n + 1)
+    dyn = want > st ? want - st : 0;
+  }
+  hipLaunchKernelGGL((k_forces_pair<NG, PA, RA, PB, RB, WAVES>), dim3(grid), dim3(kWG), dyn, s, a, b, na8, p,
                      rw.r ? *rw.r : ReceiverDev(), n_rec, rec_end, rw.half_dt);
   return hipGetLastError();
 }
@@ -862,7 +879,71 @@ hipError_t launch_pair_ng(int pa, bool ra, int pb, bool rb, const ForceArgs& a, 
   return hipErrorInvalidValue;
 }
 
+
+// Occupancy variants of the paired force launch for the autotuner (orders 4
+// to 6): level 0 is the default launch (registers budgeted for pair_waves
+// waves per SIMD, as many work-groups per CU as fit); level 1 budgets the
+// registers for at most 6 waves per SIMD and limits the launch to 6
+// work-groups per CU, level 2 to 4 (through unused dynamic LDS).  Fewer
+// resident work-groups leave each more of the L2 and more registers.  This
+// file is compiled twice, with
+// LLVM's max-ilp scheduling strategy (the library's default, which defines
+// launch_forces_pair_max_ilp and everything else) and with the default
+// scheduler (forces_default_sched.hip: the variants only).
+constexpr int occ_waves(int level, int def) {
+  return level == 0 ? def : level == 1 ? (def < 6 ? def : 6) : (def < 4 ? def : 4);
+}
+
+template <int NG, int PA, bool RA, int PB, bool RB>
+hipError_t launch_pair_occ(int level, const ForceArgs& a, const ForceArgs& b, const StepParams& p, const RecWork& w,
+                           hipStream_t s, int* g) {
+  constexpr int d = pair_waves(NG, PA, PB);
+  switch (level) {
+    case 0: return launch_pair_one<NG, PA, RA, PB, RB, occ_waves(0, d)>(a, b, p, w, s, g);
+    case 1: return launch_pair_one<NG, PA, RA, PB, RB, occ_waves(1, d)>(a, b, p, w, s, g, occ_waves(1, d));
+    case 2: return launch_pair_one<NG, PA, RA, PB, RB, occ_waves(2, d)>(a, b, p, w, s, g, occ_waves(2, d));
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int NG>
+hipError_t launch_pair_tuned_ng(int level, int pa, bool ra, int pb, bool rb, const ForceArgs& a, const ForceArgs& b,
+                                const StepParams& p, const RecWork& w, hipStream_t s, int* g) {
+  if (pa == kElasticCart && pb == kAcousticCart) {
+    if (ra && rb) return launch_pair_occ<NG, kElasticCart, true, kAcousticCart, true>(level, a, b, p, w, s, g);
+    if (ra) return launch_pair_occ<NG, kElasticCart, true, kAcousticCart, false>(level, a, b, p, w, s, g);
+    if (rb) return launch_pair_occ<NG, kElasticCart, false, kAcousticCart, true>(level, a, b, p, w, s, g);
+    return launch_pair_occ<NG, kElasticCart, false, kAcousticCart, false>(level, a, b, p, w, s, g);
+  }
+  if (pa == kElasticAxi0 && pb == kAcousticAxi0)
+    return launch_pair_occ<NG, kElasticAxi0, false, kAcousticAxi0, false>(level, a, b, p, w, s, g);
+  if (pa == kElasticAxi1 && pb == kAcousticAxi1)
+    return launch_pair_occ<NG, kElasticAxi1, false, kAcousticAxi1, false>(level, a, b, p, w, s, g);
+  return hipErrorInvalidValue;
+}
+
 }  // namespace
+
+#ifndef INSEM_FORCES_TUNED_LAUNCH
+#define INSEM_FORCES_TUNED_LAUNCH launch_forces_pair_max_ilp
+#endif
+
+hipError_t INSEM_FORCES_TUNED_LAUNCH(int ngll, int level, int pa, bool ra, int pb, bool rb, const ForceArgs& a,
+                                     const ForceArgs& b, const StepParams& p, hipStream_t s, int* grid,
+                                     const ReceiverDev* rec, int n_rec, double half_dt) {
+  RecWork w;
+  w.r = rec;
+  w.n = n_rec;
+  w.half_dt = half_dt;
+  switch (ngll) {
+    case 5: return launch_pair_tuned_ng<5>(level, pa, ra, pb, rb, a, b, p, w, s, grid);
+    case 6: return launch_pair_tuned_ng<6>(level, pa, ra, pb, rb, a, b, p, w, s, grid);
+    case 7: return launch_pair_tuned_ng<7>(level, pa, ra, pb, rb, a, b, p, w, s, grid);
+  }
+  return hipErrorInvalidValue;
+}
+
+#ifndef INSEM_FORCES_VARIANTS_ONLY
 
 bool forces_pair_supported(int pa, bool ra, int pb, bool rb) {
   const bool cart = pa == kElasticCart && pb == kAcousticCart;
@@ -870,10 +951,49 @@ bool forces_pair_supported(int pa, bool ra, int pb, bool rb) {
   return cart || (axi && !ra && !rb);
 }
 
+namespace {
+
+// The autotuner's table: the winning variant of each tuned paired launch
+// (written by in-house-sem_amd/tune/autotune.py; a missing entry is variant 0).
+// chunk: the steps per captured hipGraph chunk (0: the default).
+struct TunedForce {
+  int ngll, pa, ra, pb, rb, level, sched, chunk;
+};
+constexpr TunedForce kTunedForces[] = {
+#include "tuned_forces.inc"
+    {0, -1, 0, -1, 0, 0, 0, 0},  // end
+};
+
+const TunedForce* tuned_entry(int ngll, int pa, bool ra, int pb, bool rb) {
+  for (const TunedForce& t : kTunedForces)
+    if (t.ngll == ngll && t.pa == pa && t.ra == int(ra) && t.pb == pb && t.rb == int(rb)) return &t;
+  return nullptr;
+}
+
+}  // namespace
+
+int tuned_force_variant(int ngll, int pa, bool ra, int pb, bool rb) {
+  const TunedForce* t = tuned_entry(ngll, pa, ra, pb, rb);
+  return t ? t->level + kForceLevels * t->sched : 0;
+}
+
+int tuned_chunk_steps(int ngll, int pa, bool ra, int pb, bool rb) {
+  const TunedForce* t = tuned_entry(ngll, pa, ra, pb, rb);
+  return t ? t->chunk : 0;
+}
+
 hipError_t launch_forces_pair(int ngll, int pa, bool ra, int pb, bool rb, const ForceArgs& a, const ForceArgs& b,
                               const StepParams& p, hipStream_t s, int* grid, const ReceiverDev* rec, int n_rec,
-                              double half_dt) {
+                              double half_dt, int variant) {
   if (!p.fuse || !forces_pair_supported(pa, ra, pb, rb)) return hipErrorInvalidValue;
+  if (variant < 0) variant = tuned_force_variant(ngll, pa, ra, pb, rb);
+  if (variant < 0 || variant >= kForceVariants) return hipErrorInvalidValue;
+  if (variant != 0 && ngll >= 5 && ngll <= 7) {
+    const int level = variant % kForceLevels, sched = variant / kForceLevels;
+    return sched == 0 ? launch_forces_pair_max_ilp(ngll, level, pa, ra, pb, rb, a, b, p, s, grid, rec, n_rec, half_dt)
+                      : launch_forces_pair_default_sched(ngll, level, pa, ra, pb, rb, a, b, p, s, grid, rec, n_rec,
+                                                         half_dt);
+  }
   RecWork w;
   w.r = rec;
   w.n = n_rec;
@@ -908,5 +1028,7 @@ hipError_t launch_forces(int ngll, int phys, bool rect, bool gather, const Force
   return gather ? launch_gather_mode<true>(ngll, phys, rect, a, p, w, s, grid)
                 : launch_gather_mode<false>(ngll, phys, rect, a, p, w, s, grid);
 }
+
+#endif  // INSEM_FORCES_VARIANTS_ONLY
 
 }  // namespace insem_hip

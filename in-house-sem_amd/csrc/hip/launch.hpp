@@ -54,9 +54,29 @@ int force_batch_elements(int ngll);
 // in launch_forces.  The launch has at most 14 work-groups of padding more
 // than the two blocks' force_blocks and n_rec.
 bool forces_pair_supported(int pa, bool ra, int pb, bool rb);
+// variant: -1 the tuned one (tuned_force_variant), else level + kForceLevels
+// * sched of the autotuner's space (level: registers budgeted for the
+// default waves per SIMD, at most 6, at most 4; sched: LLVM's max-ilp or
+// default scheduling strategy).  Variants other than 0 exist for orders 4 to
+// 6 (ngll 5 to 7); elsewhere every variant runs variant 0.  All variants give
+// the same bits.
+constexpr int kForceLevels = 3;
+constexpr int kForceVariants = 2 * kForceLevels;
 hipError_t launch_forces_pair(int ngll, int pa, bool ra, int pb, bool rb, const ForceArgs& a, const ForceArgs& b,
                               const StepParams& p, hipStream_t s, int* grid, const ReceiverDev* rec, int n_rec,
-                              double half_dt);
+                              double half_dt, int variant = -1);
+int tuned_force_variant(int ngll, int pa, bool ra, int pb, bool rb);
+// The tuned steps per captured chunk of a model with this paired launch (0:
+// none recorded).
+int tuned_chunk_steps(int ngll, int pa, bool ra, int pb, bool rb);
+// The variants of one scheduling strategy (level as above).
+hipError_t launch_forces_pair_max_ilp(int ngll, int level, int pa, bool ra, int pb, bool rb, const ForceArgs& a,
+                                      const ForceArgs& b, const StepParams& p, hipStream_t s, int* grid,
+                                      const ReceiverDev* rec, int n_rec, double half_dt);
+hipError_t launch_forces_pair_default_sched(int ngll, int level, int pa, bool ra, int pb, bool rb,
+                                            const ForceArgs& a, const ForceArgs& b, const StepParams& p,
+                                            hipStream_t s, int* grid, const ReceiverDev* rec, int n_rec,
+                                            double half_dt);
 
 struct StepCoef {
   float rdt = 0.0f, hdt = 0.0f, hdt2 = 0.0f;  // float(dt), float(dt/2), float(dt^2/2)

@@ -205,7 +205,7 @@ class ImageBuilder {
       d.point_x.resize(ne * npe_);
       d.point_z.resize(ne * npe_);
       for (std::size_t e = 0; e < ne; ++e)
-        for (int k = 0; k < npe_; ++k) {
+        for (std::size_t k = 0; k < npe_; ++k) {
           const std::size_t p = static_cast<std::size_t>(d.leaves[e]) * npe_ + k;
           d.point_x[e * npe_ + k] = G_.x[p];
           d.point_z[e * npe_ + k] = G_.z[p];

@@ -58,6 +58,10 @@ def hip():
         _declare(lib, "insem_hip_get_field", c_int, c_void, c_int, c_int, P(ctypes.c_float))
         _declare(lib, "insem_hip_advance", c_int, c_void, c_int)
         _declare(lib, "insem_hip_prepare", c_int, c_void, c_int)
+        _declare(lib, "insem_hip_set_force_variant", c_int, c_void, c_int)
+        _declare(lib, "insem_hip_force_variant", c_int, c_void, P(c_int), P(c_int))
+        _declare(lib, "insem_hip_set_chunk", c_int, c_void, c_int)
+        _declare(lib, "insem_hip_chunk", c_int, c_void, P(c_int))
         _declare(lib, "insem_hip_step", c_int, c_void, P(c_int))
         _declare(lib, "insem_hip_n_samples", c_int, c_void, P(c_int))
         _declare(lib, "insem_hip_read_traces", c_int, c_void, P(c_double))

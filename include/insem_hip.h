@@ -234,6 +234,29 @@ int insem_hip_advance(insem_hip_ctx* ctx, int32_t n);
  * timed region, and a serving loop before its first chunk. */
 int insem_hip_prepare(insem_hip_ctx* ctx, int32_t n);
 
+/* Autotuner variants of the paired element-force launch of a time step (no
+ * counterpart in the reference, whose kernels have one shape).  variant -1
+ * (the default) takes the winner that the committed tuning table records for
+ * the model's order and element kinds; 0 .. 5 force one variant: level
+ * (variant % 3) budgets the kernel's registers for the default waves per SIMD,
+ * at most 6 or at most 4, and variant / 3 picks LLVM's max-ilp (0) or
+ * default (1) scheduling strategy.  Orders 4 to 6 have all six variants,
+ * other orders run variant 0.  Every variant gives the same bits; a change
+ * drops the captured step chunks.  The environment variable
+ * INSEM_HIP_FORCE_VARIANT sets the initial value. */
+int insem_hip_set_force_variant(insem_hip_ctx* ctx, int32_t variant);
+/* The variant the context's paired launch runs, and whether the model has a
+ * paired launch at all (paired = 0: single-domain or batch-less models whose
+ * first blocks have no paired kernel; variant is then 0). */
+int insem_hip_force_variant(insem_hip_ctx* ctx, int32_t* variant, int32_t* paired);
+/* Steps per captured hipGraph chunk of insem_hip_advance (no counterpart in
+ * the reference): 1 .. 1024 fixes it, 0 returns to the tuned table's value
+ * for the model's order and element kinds (32 where it records none).  The
+ * chunk length changes no result bit; a change drops the captured chunks.
+ * The environment variable INSEM_HIP_CHUNK sets the initial value. */
+int insem_hip_set_chunk(insem_hip_ctx* ctx, int32_t steps);
+int insem_hip_chunk(insem_hip_ctx* ctx, int32_t* steps);
+
 /* Steps done. */
 int insem_hip_step(insem_hip_ctx* ctx, int32_t* step);
 
