@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -182,6 +183,14 @@ struct insem_hip_ctx {
   std::map<int, std::pair<double, int>> block_ms;
   int cur_tag = 0;
   std::vector<hipEvent_t> ev_pool;
+  // progress of queued advance calls: (event recorded after the call's
+  // launches, the step count it reaches), oldest first; the steps the device
+  // has completed as far as finished events show
+  std::deque<std::pair<hipEvent_t, int>> progress;
+  std::vector<hipEvent_t> progress_pool;
+  int device_step = 0;
+  bool device_started = false;    // an advance call has finished on the device
+  hipStream_t copy_stream = nullptr;  // reads of finished trace rows (insem_hip_read_traces_done)
   double force_ms = 0.0, update_ms = 0.0;
   int force_n = 0, update_n = 0;
 
@@ -303,6 +312,11 @@ void free_all(insem_hip_ctx* c) {
     (void)hipEventDestroy(e.second);
   }
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (auto& e : c->progress) (void)hipEventDestroy(e.first);
+  c->progress.clear();
+  for (hipEvent_t e : c->progress_pool) (void)hipEventDestroy(e);
+  c->progress_pool.clear();
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -1178,6 +1192,29 @@ void permute_out(const DomainRT& R, const std::vector<float>& src, float* out) {
     for (int64_t gn = 0; gn < R.ng; ++gn) out[cc * R.ng + R.perm_old[gn]] = src[cc * R.ng + gn];
 }
 
+// Advance markers: the host may queue at most this many advance calls ahead
+// of the device before an advance waits for the oldest.
+constexpr std::size_t kMaxMarkers = 4096;
+
+// Retires the markers of finished advance calls (the device step follows
+// them); with wait, also waits for the oldest unfinished one.
+void retire_markers(insem_hip_ctx* c, bool wait) {
+  while (!c->progress.empty()) {
+    hipEvent_t e = c->progress.front().first;
+    hipError_t q = hipEventQuery(e);
+    if (q == hipErrorNotReady) {
+      if (!wait) break;
+      q = hipEventSynchronize(e);
+      wait = false;
+    }
+    HIP_TRY(q);
+    c->device_step = c->progress.front().second;
+    c->device_started = true;
+    c->progress_pool.push_back(e);
+    c->progress.pop_front();
+  }
+}
+
 }  // namespace
 
 // ===================================================================== ABI
@@ -1555,6 +1592,27 @@ int insem_hip_advance(insem_hip_ctx* c, int32_t n) {
     const int m = std::max(0, std::min(n, c->n_steps - c->step));
     for_each_chunk(c, m, [&](int k, bool first, bool last, int level) { run_chunk(c, k, first, last, level); });
     c->step += m;
+    // a marker of this call's end, for insem_hip_progress; markers of
+    // finished calls are retired here too, and a host that queues more than
+    // kMaxMarkers calls ahead of the device waits for the oldest
+    retire_markers(c, c->progress.size() >= kMaxMarkers);
+    hipEvent_t e = nullptr;
+    if (!c->progress_pool.empty()) {
+      e = c->progress_pool.back();
+      c->progress_pool.pop_back();
+    } else {
+      HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventRecord(e, c->stream));
+    c->progress.emplace_back(e, c->step);
+  });
+}
+
+int insem_hip_progress(insem_hip_ctx* c, int32_t* step) {
+  if (!c || !step) return set_error(INSEM_HIP_EINVAL, "null argument");
+  return guarded(c, [&] {
+    retire_markers(c, false);
+    *step = c->device_step;
   });
 }
 
@@ -1628,28 +1686,63 @@ int insem_hip_n_samples(insem_hip_ctx* c, int32_t* n) {
   return INSEM_HIP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The trace rows [0, ns) copied to the host h (on the context's stream, or
+// on its copy stream without waiting for the context's stream), arranged as
+// insem_hip_read_traces documents.
+void format_traces(insem_hip_ctx* c, int ns, bool side, double* out) {
+  const int nc = c->rec.n_cols;
+  std::vector<double> h(static_cast<size_t>(ns) * std::max(1, nc));
+  if (ns && nc) {
+    if (side) {
+      if (!c->copy_stream) HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+      HIP_TRY(hipMemcpyAsync(h.data(), c->traces, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->copy_stream));
+      HIP_TRY(hipStreamSynchronize(c->copy_stream));
+    } else {
+      c->d2h(h.data(), c->traces, h.size() * sizeof(double));
+    }
+  }
+  const double kh = 0.70710678118654752440;
+  size_t o = 0;
+  for (const TraceCol& tc : c->cols) {
+    for (int s = 0; s < ns; ++s) {
+      const double* row = h.data() + static_cast<size_t>(s) * nc + tc.col;
+      if (tc.m1_elastic && tc.raw_ncomp == 3) {
+        const double up = row[0], um = row[1];
+        out[o++] = kh * (up + um);
+        out[o++] = kh * (up - um);
+        out[o++] = row[2];
+      } else {
+        for (int k = 0; k < tc.raw_ncomp; ++k) out[o++] = row[k];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
 int insem_hip_read_traces(insem_hip_ctx* c, double* out) {
   if (!out) return set_error(INSEM_HIP_EINVAL, "null output");
   return guarded(c, [&] {
     HIP_TRY(hipStreamSynchronize(c->stream));
-    const int ns = c->initialized ? c->step / c->record_every + 1 : 0, nc = c->rec.n_cols;
-    std::vector<double> h(static_cast<size_t>(ns) * std::max(1, nc));
-    if (ns && nc) c->d2h(h.data(), c->traces, h.size() * sizeof(double));
-    const double kh = 0.70710678118654752440;
-    size_t o = 0;
-    for (const TraceCol& tc : c->cols) {
-      for (int s = 0; s < ns; ++s) {
-        const double* row = h.data() + static_cast<size_t>(s) * nc + tc.col;
-        if (tc.m1_elastic && tc.raw_ncomp == 3) {
-          const double up = row[0], um = row[1];
-          out[o++] = kh * (up + um);
-          out[o++] = kh * (up - um);
-          out[o++] = row[2];
-        } else {
-          for (int k = 0; k < tc.raw_ncomp; ++k) out[o++] = row[k];
-        }
-      }
-    }
+    format_traces(c, c->initialized ? c->step / c->record_every + 1 : 0, false, out);
+  });
+}
+
+int insem_hip_read_traces_done(insem_hip_ctx* c, double* out, int32_t* n_samples) {
+  if (!out || !n_samples) return set_error(INSEM_HIP_EINVAL, "null output");
+  return guarded(c, [&] {
+    retire_markers(c, false);
+    // rows of finished advance calls are written and no queued launch
+    // writes them again
+    const int ns = c->device_started ? c->device_step / c->record_every + 1 : 0;
+    format_traces(c, ns, true, out);
+    *n_samples = ns;
   });
 }
 

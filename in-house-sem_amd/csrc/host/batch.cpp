@@ -202,6 +202,20 @@ double PowerJob::lambda() {
   return lambda_;
 }
 
+insem_hip_ctx* PowerJob::take_context(double* lambda) {
+  INSEM_REQUIRE(ctx_, "the power iteration's context was already released");
+  const int rc = insem_hip_lambda_result(ctx_, &lambda_);
+  if (rc != INSEM_HIP_OK) {
+    insem_hip_destroy(ctx_);
+    ctx_ = nullptr;
+    check_hip(rc);
+  }
+  insem_hip_ctx* c = ctx_;
+  ctx_ = nullptr;
+  *lambda = lambda_;
+  return c;
+}
+
 BatchSolver::BatchSolver(std::vector<std::unique_ptr<SetupImage>> models) : models_(std::move(models)) {
   INSEM_REQUIRE(!models_.empty(), "a batch needs at least one model");
   for (const auto& m : models_) INSEM_REQUIRE(m, "null model in a batch");
@@ -300,16 +314,11 @@ int BatchSolver::n_samples(int m) const {
   return std::min(static_cast<int>(ns), own);
 }
 
-std::vector<Solver::Trace> BatchSolver::traces(int m) const {
-  int32_t ns = 0;
-  check_hip(insem_hip_n_samples(ctx_, &ns));
+// Model m's traces from the union's trace rows [0, ns) (all receivers'
+// quantities, as insem_hip_read_traces lays them out).
+std::vector<Solver::Trace> BatchSolver::model_traces(const std::vector<double>& all, int ns, int m) const {
   const SetupImage& U = *union_.image;
-  std::size_t total = 0;
-  for (const auto& r : U.receivers)
-    for (int nc : r.ncomp) total += static_cast<std::size_t>(nc) * ns;
-  std::vector<double> all(total + 1);
-  check_hip(insem_hip_read_traces(ctx_, all.data()));
-  const int keep = n_samples(m);
+  const int keep = std::min(ns, n_steps(m) / U.time.record_every + 1);
   std::vector<Solver::Trace> out;
   std::size_t o = 0;
   const int u = slot(m);
@@ -331,6 +340,38 @@ std::vector<Solver::Trace> BatchSolver::traces(int m) const {
     }
   }
   return out;
+}
+
+std::size_t BatchSolver::trace_values(int ns) const {
+  std::size_t total = 0;
+  for (const auto& r : union_.image->receivers)
+    for (int nc : r.ncomp) total += static_cast<std::size_t>(nc) * ns;
+  return total;
+}
+
+std::vector<Solver::Trace> BatchSolver::traces(int m) const {
+  int32_t ns = 0;
+  check_hip(insem_hip_n_samples(ctx_, &ns));
+  std::vector<double> all(trace_values(ns) + 1);
+  check_hip(insem_hip_read_traces(ctx_, all.data()));
+  return model_traces(all, ns, m);
+}
+
+int BatchSolver::device_step() const {
+  int32_t s = 0;
+  check_hip(insem_hip_progress(ctx_, &s));
+  return s;
+}
+
+std::vector<Solver::Trace> BatchSolver::traces_done(int m) const {
+  int32_t ns = 0;
+  check_hip(insem_hip_n_samples(ctx_, &ns));  // at least the finished rows
+  std::vector<double> all(trace_values(ns) + 1);
+  int32_t done = 0;
+  check_hip(insem_hip_read_traces_done(ctx_, all.data(), &done));
+  INSEM_REQUIRE(done >= n_steps(m) / union_.image->time.record_every + 1,
+                "model " << m << " of the batch has not finished its steps on the device");
+  return model_traces(all, done, m);
 }
 
 int64_t BatchSolver::footprint() const {

@@ -44,6 +44,10 @@ class PowerJob {
   PowerJob& operator=(const PowerJob&) = delete;
   bool ready() const;
   double lambda();
+  // Waits for lambda_max and hands the context over (for a Solver of the
+  // model: the context is the one that Solver would build); the job keeps
+  // no context afterwards.
+  insem_hip_ctx* take_context(double* lambda);
 
  private:
   insem_hip_ctx* ctx_ = nullptr;
@@ -77,10 +81,17 @@ class BatchSolver {
   int n_samples(int m) const;
   // Model m's traces, as Solver::traces of that model alone.
   std::vector<Solver::Trace> traces(int m) const;
+  // Steps the device has completed (does not wait), and model m's traces
+  // read without waiting for the steps still queued (its last step must be
+  // done on the device).
+  int device_step() const;
+  std::vector<Solver::Trace> traces_done(int m) const;
   insem_hip_ctx* context() const { return ctx_; }
   int64_t footprint() const;
 
  private:
+  std::vector<Solver::Trace> model_traces(const std::vector<double>& all, int ns, int m) const;
+  std::size_t trace_values(int ns) const;
   void start(const std::vector<double>& lambdas);
   int slot(int m) const { return slot_.at(m); }
 

@@ -504,6 +504,22 @@ int insem_power_lambda(void* job, double* lambda) {
 
 void insem_power_free(void* job) { delete static_cast<PowerJob*>(job); }
 
+// A Solver of the model on the job's own context (the job must have been
+// started from this image, which the solver takes over).
+void* insem_power_solver(void* job, void* image) {
+  return guard_ptr([&]() -> void* {
+    std::unique_ptr<SetupImage> img(static_cast<SetupImage*>(image));
+    double lambda = 0.0;
+    insem_hip_ctx* ctx = static_cast<PowerJob*>(job)->take_context(&lambda);
+    try {
+      return new Solver(std::move(img), ctx, lambda);
+    } catch (...) {
+      insem_hip_destroy(ctx);
+      throw;
+    }
+  });
+}
+
 void* insem_batch_context(void* h) { return static_cast<BatchSolver*>(h)->context(); }
 
 void* insem_batch_model_image(void* h, int32_t m) {
@@ -542,6 +558,22 @@ int insem_batch_traces(void* h, int32_t m, double* out) {
       o += t.data.size();
     }
   });
+}
+
+int insem_batch_traces_done(void* h, int32_t m, double* out) {
+  return guard([&] {
+    const BatchSolver& b = *static_cast<BatchSolver*>(h);
+    INSEM_REQUIRE(m >= 0 && m < b.n_models(), "no model " << m << " in the batch");
+    std::size_t o = 0;
+    for (const auto& t : b.traces_done(m)) {
+      std::memcpy(out + o, t.data.data(), t.data.size() * sizeof(double));
+      o += t.data.size();
+    }
+  });
+}
+
+int insem_batch_device_step(void* h, int32_t* step) {
+  return guard([&] { *step = static_cast<BatchSolver*>(h)->device_step(); });
 }
 
 int64_t insem_batch_footprint(void* h) {

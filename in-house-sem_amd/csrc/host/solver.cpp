@@ -41,25 +41,35 @@ Solver::Solver(std::unique_ptr<SetupImage> image) : img_(std::move(image)) {
   INSEM_REQUIRE(img_, "solver needs a setup");
   check_hip(insem_hip_create(&img_->view, &ctx_));
   try {
-    const ModelTime m = model_time(img_->time, device_lambda_max(ctx_, *img_));
-    stable_dt_ = m.stable_dt;
-    dt_ = m.dt;
-    n_steps_ = m.n_steps;
-    const std::vector<float> amps = img_->source_amplitudes(dt_, n_steps_);
-    check_hip(insem_hip_set_time(ctx_, dt_, n_steps_, amps.data()));
-    if (img_->time.snapshot_every > 0)
-      for (const std::string& q : img_->time.snapshot_quantities)
-        for (const auto& d : img_->domains) {
-          const bool has = d.medium == Medium::Elastic
-                               ? (q == "displacement" || q == "velocity" || q == "acceleration")
-                               : q == "pressure";
-          if (has) snapshots_.push_back({d.medium, q, d.ncomp, {}});
-        }
+    set_up_time(device_lambda_max(ctx_, *img_));
   } catch (...) {
     insem_hip_destroy(ctx_);
     ctx_ = nullptr;
     throw;
   }
+}
+
+Solver::Solver(std::unique_ptr<SetupImage> image, insem_hip_ctx* ctx, double lambda)
+    : img_(std::move(image)), ctx_(ctx) {
+  INSEM_REQUIRE(img_ && ctx_, "solver needs a setup and its context");
+  set_up_time(lambda);
+}
+
+void Solver::set_up_time(double lambda) {
+  const ModelTime m = model_time(img_->time, lambda);
+  stable_dt_ = m.stable_dt;
+  dt_ = m.dt;
+  n_steps_ = m.n_steps;
+  const std::vector<float> amps = img_->source_amplitudes(dt_, n_steps_);
+  check_hip(insem_hip_set_time(ctx_, dt_, n_steps_, amps.data()));
+  if (img_->time.snapshot_every > 0)
+    for (const std::string& q : img_->time.snapshot_quantities)
+      for (const auto& d : img_->domains) {
+        const bool has = d.medium == Medium::Elastic
+                             ? (q == "displacement" || q == "velocity" || q == "acceleration")
+                             : q == "pressure";
+        if (has) snapshots_.push_back({d.medium, q, d.ncomp, {}});
+      }
 }
 
 Solver::~Solver() {

@@ -21,6 +21,11 @@ namespace insem_amd {
 class Solver {
  public:
   explicit Solver(std::unique_ptr<SetupImage> image);
+  // The solver of a context already created from this image whose power
+  // iteration gave lambda (a finished PowerJob's): the same state as the
+  // constructor above reaches, without building the context again.  Takes
+  // the context over.
+  Solver(std::unique_ptr<SetupImage> image, insem_hip_ctx* ctx, double lambda);
   ~Solver();
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
@@ -68,6 +73,7 @@ class Solver {
 
  private:
   std::unique_ptr<SetupImage> img_;
+  void set_up_time(double lambda);  // dt, n_steps, source amplitudes, snapshots
   void record_snapshot();
   const SetupImage::Domain& domain_of(Medium m) const;
 

@@ -61,6 +61,7 @@ def hip():
         _declare(lib, "insem_hip_set_force_variant", c_int, c_void, c_int)
         _declare(lib, "insem_hip_force_variant", c_int, c_void, P(c_int), P(c_int))
         _declare(lib, "insem_hip_set_chunk", c_int, c_void, c_int)
+        _declare(lib, "insem_hip_progress", c_int, c_void, P(c_int))
         _declare(lib, "insem_hip_chunk", c_int, c_void, P(c_int))
         _declare(lib, "insem_hip_step", c_int, c_void, P(c_int))
         _declare(lib, "insem_hip_n_samples", c_int, c_void, P(c_int))
@@ -152,11 +153,14 @@ def host():
         d("insem_power_ready", c_int, c_void, P(c_int))
         d("insem_power_lambda", c_int, c_void, P(c_double))
         d("insem_power_free", None, c_void)
+        d("insem_power_solver", c_void, c_void, c_void)
         d("insem_batch_context", c_void, c_void)
         d("insem_batch_model_image", c_void, c_void, c_int)
         d("insem_batch_info", c_int, c_void, c_int, P(c_double), P(c_int))
         d("insem_batch_advance", c_int, c_void, c_int)
         d("insem_batch_traces", c_int, c_void, c_int, P(c_double))
+        d("insem_batch_traces_done", c_int, c_void, c_int, P(c_double))
+        d("insem_batch_device_step", c_int, c_void, P(c_int))
         d("insem_batch_footprint", c_i64, c_void)
         _host = lib
     return _host

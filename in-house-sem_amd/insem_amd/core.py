@@ -466,6 +466,19 @@ class PowerJob:
         check(_lib.host().insem_power_lambda(self._h, ctypes.byref(lam)))
         return lam.value
 
+    def into_solver(self, image: "SetupImage") -> "Solver":
+        """The model's Solver on this job's own context (the image the job
+        was started from, which the solver takes over): the state Solver(image)
+        reaches, without building the context a second time.  The job keeps
+        no context afterwards."""
+        lib = _lib.host()
+        image._owned = False
+        s = Solver.__new__(Solver)
+        s._handle = check_ptr(lib.insem_power_solver(self._h, image._h))
+        image._h = None
+        s.image = SetupImage(lib.insem_solver_image(s._handle), owned=False)
+        return s
+
     def close(self):
         h = getattr(self, "_h", None)
         if h:
@@ -567,6 +580,26 @@ class BatchSolver:
         flag = ctypes.c_int32()
         check(_lib.hip().insem_hip_query(ctypes.c_void_p(self.context), ctypes.byref(flag)))
         return bool(flag.value)
+
+    def device_step(self) -> int:
+        """Steps the device has completed (does not block)."""
+        n = ctypes.c_int32()
+        check(_lib.host().insem_batch_device_step(self._h, ctypes.byref(n)))
+        return n.value
+
+    def traces_done(self, m: int) -> Dict[Tuple[str, str], np.ndarray]:
+        """Model m's traces (as traces(m)) read without waiting for the steps
+        still queued; its last step must be done on the device
+        (device_step() >= n_steps(m))."""
+        ns = int(self._info(m)[1][1])
+        img = self.models[m]
+        buf = np.zeros(max(sum(nc for _, _, nc in img.traces) * ns, 1))
+        check(_lib.host().insem_batch_traces_done(self._h, m, _dptr(buf)))
+        out, o = {}, 0
+        for name, q, nc in img.traces:
+            out[(name, q)] = buf[o:o + nc * ns].reshape(ns, nc).copy()
+            o += nc * ns
+        return out
 
     def traces(self, m: int) -> Dict[Tuple[str, str], np.ndarray]:
         """Model m's traces, keyed and shaped as Solver.traces of that model."""

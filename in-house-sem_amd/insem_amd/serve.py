@@ -12,16 +12,23 @@ Requests (a Simulation each) arrive over time and go through four stages:
 3. admission: requests whose time step is known wait for a slot.  Requests
    of at least `large_dofs` degrees of freedom (the 2D layered models) have
    a slot class of their own (`max_large_slots` slots, one request each),
-   so that they neither hold up nor wait for the borehole batches.  When a
-   small slot is free, the waiting small requests that can share a batch
-   with the oldest one (same polynomial order, coordinate system, azimuthal
-   order, media and record cadence; at most `max_batch`) are admitted
-   together as one BatchSolver, within the device-memory budget; among
-   those, the ones whose step counts are closest to the oldest one's come
-   first (a batch runs until its longest model ends);
-4. stepping: each slot is kept busy by queueing its next chunk of steps as
-   soon as its stream is idle.  A request's traces are returned as soon as
-   its own step count is reached (bitwise those of the request run alone;
+   so that they neither hold up nor wait for the borehole batches; such a
+   request runs as a plain Solver on the context its power iteration
+   already built.  When a small slot is free, the waiting small requests
+   that can share a batch (same polynomial order, coordinate system,
+   azimuthal order, media and record cadence; at most `max_batch`) are
+   admitted together as one BatchSolver, within the device-memory budget:
+   the oldest key whose group is full, or whose oldest request has waited
+   `batch_wait` seconds since its time step was known, or that can get no
+   more requests (none are still arriving or being set up); among the
+   group, the requests whose step counts are closest to the oldest one's
+   come first (a batch runs until its longest model ends);
+4. stepping: each slot keeps `lookahead` chunks of steps queued on its
+   stream (never past the last step of its longest unfinished model), so
+   that the device does not wait for the loop; progress is read from
+   markers of the queued calls without blocking.  A request's traces are
+   read as soon as the device has done its own last step, without waiting
+   for the batch's queued steps (bitwise those of the request run alone;
    with their slowness-time coherence for borehole arrays), while the rest
    of its batch runs on.
 
@@ -63,6 +70,8 @@ class Request:
     n_dofs: int = 0
     large: bool = False
     lam: float = float("nan")           # lambda_max of its power iteration
+    ready: float = float("nan")         # when its time step was known
+    job: object = None                  # a large request's finished power job (its context)
     admitted: float = float("nan")
     done: float = float("nan")
     dt: float = float("nan")
@@ -108,6 +117,48 @@ def device_memory() -> Tuple[int, int]:
     return f.value, t.value
 
 
+class _Solo:
+    """A large request's Solver behind the slot interface of a batch of one
+    model (the slot runs it as the request alone would run)."""
+
+    def __init__(self, solver):
+        self.solver = solver
+        self.models = [solver.image]
+
+    def dt(self, m: int) -> float:
+        return self.solver.dt
+
+    def n_steps(self, m: int) -> int:
+        return self.solver.n_steps
+
+    @property
+    def step(self) -> int:
+        return self.solver.step
+
+    def idle(self) -> bool:
+        flag = ctypes.c_int32()
+        _lib.check_hip(_lib.hip().insem_hip_query(ctypes.c_void_p(self.solver.context), ctypes.byref(flag)))
+        return bool(flag.value)
+
+    def device_step(self) -> int:
+        n = ctypes.c_int32()
+        _lib.check_hip(_lib.hip().insem_hip_progress(ctypes.c_void_p(self.solver.context), ctypes.byref(n)))
+        return n.value
+
+    def advance(self, n: int) -> None:
+        self.solver.advance(n)
+
+    def traces(self, m: int):
+        return self.solver.traces
+
+    def traces_done(self, m: int):
+        # the solver never has steps queued past its last one
+        return self.solver.traces
+
+    def close(self) -> None:
+        self.solver.close()
+
+
 @dataclass
 class _Slot:
     batch: BatchSolver
@@ -115,6 +166,7 @@ class _Slot:
     footprint: int
     large: bool
     open: List[int] = field(default_factory=list)   # models whose traces are still to be returned
+    held: bool = False  # its bytes were counted in `used` before admission (a large request's context)
 
     def idle(self) -> bool:
         return self.batch.idle()
@@ -133,12 +185,16 @@ class Server:
     offsets are the receivers' z), or None.  timeout: seconds after its
     arrival by which a request must have finished, or None."""
 
-    def __init__(self, max_slots: int = 4, max_batch: int = 16, chunk: int = 1024,
+    def __init__(self, max_slots: int = 4, max_batch: int = 16, chunk: int = 1024, lookahead: int = 2,
                  hbm_budget: Optional[int] = None, stc: Optional[dict] = None, max_large_slots: int = 1,
                  large_dofs: int = 2_000_000, max_power_jobs: int = 4, timeout: Optional[float] = None,
-                 workers: int = 4):
+                 workers: int = 4, batch_wait: float = 0.0):
         if max_slots < 1 or max_batch < 1 or chunk < 1 or max_large_slots < 0 or max_power_jobs < 1:
             raise ValueError("slots, batch size, chunk and power jobs must be positive")
+        if batch_wait < 0 or lookahead < 1:
+            raise ValueError("batch_wait must be at least 0 and lookahead at least 1")
+        self.lookahead = lookahead
+        self.batch_wait = batch_wait
         self.max_slots = max_slots
         self.max_batch = max_batch
         self.chunk = chunk
@@ -173,6 +229,9 @@ class Server:
     def _batch(self, images: List[SetupImage], lambdas: List[float]):
         return BatchSolver(images, lambdas)
 
+    def _solo(self, job, image: SetupImage):
+        return _Solo(job.into_solver(image))
+
     def _prepare(self, r: Request) -> Request:
         r.image = self._setup(r)
         r.key = batch_key(r.image)
@@ -181,12 +240,30 @@ class Server:
         r.large = self.large_dofs is not None and r.n_dofs >= self.large_dofs
         return r
 
-    def _admit(self, queue: deque, large: bool, t0: float, used: int):
-        """A batch of the waiting requests of one class; None when none fits
-        the memory left, _FAILED when the batch could not be set up (its
-        requests then fail)."""
-        head = queue[0]
-        limit = 1 if large else self.max_batch
+    def _admit(self, queue: deque, large: bool, t0: float, used: int, head: Optional[Request] = None):
+        """A batch of the waiting requests of one class, led by `head` (default
+        the oldest); None when none fits the memory left, _FAILED when the
+        batch could not be set up (its requests then fail).  A large request
+        (whose power iteration's context holds its model already: its bytes
+        are in `used`) becomes a Solver on that context."""
+        head = queue[0] if head is None else head
+        if large:
+            queue.remove(head)
+            head.admitted = time.perf_counter() - t0
+            img, job = head.image, head.job
+            head.image = head.job = None
+            try:
+                with self._timed("batch_setup_s"):
+                    b = self._solo(job, img)
+            except Exception as e:  # noqa: BLE001 - the request fails, the loop goes on
+                self._fail(head, f"set-up failed: {e}", t0)
+                return _FAILED
+            finally:
+                job.close()
+            head.dt, head.n_steps = b.dt(0), b.n_steps(0)
+            self.events.append((time.perf_counter() - t0, "admit", [head.rid], [head.n_steps]))
+            return _Slot(b, [head], head.footprint, True, [0], held=True)
+        limit = self.max_batch
         # the oldest request, then those that can share its batch by how
         # close their step counts are to its own (a batch runs as long as
         # its longest model), oldest first among equals
@@ -221,6 +298,20 @@ class Server:
         self.events.append((time.perf_counter() - t0, "admit", [r.rid for r in group], [r.n_steps for r in group]))
         return _Slot(b, group, need, large, list(range(len(group))))
 
+    def _batch_head(self, queue: deque, now: float, upstream: bool) -> Optional[Request]:
+        """The oldest request of the first key (by its oldest request) whose
+        group is ready to run: full, or waited batch_wait seconds, or with no
+        more requests to come; None when every group is still gathering."""
+        seen = {}
+        for r in queue:
+            if r.key not in seen:
+                seen[r.key] = [r, 0]
+            seen[r.key][1] += 1
+        for head, n in seen.values():
+            if n >= self.max_batch or not upstream or now - head.ready >= self.batch_wait:
+                return head
+        return None
+
     def _coherence(self, r: Request, step: float) -> None:
         q = r.sim.receivers[0].quantities[0]
         recs = [rec for rec in r.sim.receivers if (rec.name, q) in r.traces]
@@ -234,7 +325,7 @@ class Server:
         b, r = slot.batch, slot.requests[m]
         if r.error is not None:
             return None
-        r.traces = b.traces(m)
+        r.traces = b.traces_done(m)
         if not all(np.isfinite(v).all() for v in r.traces.values()):
             r.error = "non-finite traces: the model diverged"
             r.done = time.perf_counter() - t0
@@ -311,6 +402,14 @@ class Server:
                     if not job.ready():
                         continue
                     power.remove((r, job))
+                    if r.large:
+                        # the job's context becomes the request's solver: its
+                        # bytes stay counted until the slot closes
+                        r.job = job
+                        r.ready = now
+                        queues[True].append(r)
+                        busy = True
+                        continue
                     used -= r.footprint
                     try:
                         r.lam = job.lambda_max()
@@ -321,33 +420,41 @@ class Server:
                         continue
                     finally:
                         job.close()
-                    queues[r.large].append(r)
+                    r.ready = now
+                    queues[False].append(r)
                     busy = True
                 # requests past their deadline before admission
                 for q in queues.values():
                     for r in [r for r in q if expired(r, now)]:
                         q.remove(r)
                         r.image = None
+                        if r.job is not None:
+                            r.job.close()
+                            r.job = None
+                            used -= r.footprint
                         self._fail(r, f"timed out after {self.timeout:g} s waiting for a slot", t0)
                 # 3. admission, per slot class
+                upstream = bool(pending or building or timing or power)
                 for large, cap in ((True, self.max_large_slots), (False, self.max_slots)):
                     q = queues[large]
                     while q and sum(s.large == large for s in slots) < cap:
-                        slot = self._admit(q, large, t0, used)
+                        head = q[0] if large else self._batch_head(q, now, upstream)
+                        if head is None:
+                            break  # every key's group is still gathering requests
+                        slot = self._admit(q, large, t0, used, head)
                         if slot is _FAILED:
                             continue
                         if slot is None:
                             break
                         slots.append(slot)
-                        used += slot.footprint
+                        if not slot.held:
+                            used += slot.footprint
                         busy = True
                 # 4. stepping and results
                 for slot in list(slots):
-                    if not slot.idle():
-                        continue
                     b = slot.batch
-                    step = b.step
-                    for m in [m for m in slot.open if slot.requests[m].n_steps <= step]:
+                    done = b.device_step()
+                    for m in [m for m in slot.open if slot.requests[m].n_steps <= done]:
                         slot.open.remove(m)
                         with self._timed("deliver_s"):
                             j = self._deliver(slot, m, pool, t0)
@@ -357,12 +464,14 @@ class Server:
                         slot.open.remove(m)
                         self._fail(slot.requests[m], f"timed out after {self.timeout:g} s", t0)
                     if not slot.open:
-                        b.close()
+                        b.close()  # waits for steps still queued (of finished or expired models)
                         slots.remove(slot)
                         used -= slot.footprint
-                    else:
+                        continue
+                    last = max(slot.requests[m].n_steps for m in slot.open)
+                    while b.step < last and b.step - done < self.lookahead * self.chunk:
                         with self._timed("advance_s"):
-                            b.advance(self.chunk)
+                            b.advance(min(self.chunk, last - b.step))
                         busy = True
                 for j in [j for j in jobs if j.done()]:
                     j.result()

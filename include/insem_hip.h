@@ -260,6 +260,19 @@ int insem_hip_chunk(insem_hip_ctx* ctx, int32_t* steps);
 /* Steps done. */
 int insem_hip_step(insem_hip_ctx* ctx, int32_t* step);
 
+/* Steps the device has completed, without waiting: the step count reached by
+ * the latest insem_hip_advance call whose launches have all finished (no
+ * counterpart in the reference, whose advance returns when its steps are
+ * done).  A caller that keeps several advance calls queued (a serving loop)
+ * reads results of models whose last step is at most this. */
+int insem_hip_progress(insem_hip_ctx* ctx, int32_t* step);
+/* The trace rows of the steps the device has completed (insem_hip_progress),
+ * laid out as insem_hip_read_traces lays out all rows, copied without waiting
+ * for the advance calls still queued (no counterpart in the reference); the
+ * row count goes to n_samples.  A serving loop reads a batch model's traces
+ * this way as soon as the model's last step is done. */
+int insem_hip_read_traces_done(insem_hip_ctx* ctx, double* out, int32_t* n_samples);
+
 /*
  * Receiver traces.  n_samples = samples recorded so far.  For receiver r and
  * quantity q the trace is [n_samples][ncomp] doubles, concatenated in

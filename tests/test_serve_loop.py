@@ -79,6 +79,12 @@ class FakeBatch:
     def idle(self):
         return True
 
+    def device_step(self):
+        return self.step
+
+    def traces_done(self, m):
+        return self.traces(m)
+
     def advance(self, n):
         assert not self.closed
         time.sleep(1e-4)  # a device call releases the interpreter lock, as this does
@@ -113,6 +119,10 @@ class FakeServer(serve.Server):
         if any(im.spec.get("batch_fails") for im in images):
             raise RuntimeError("out of memory")
         return FakeBatch(images, lambdas, self.log)
+
+    def _solo(self, job, image):
+        # a large request runs alone on its power job's context
+        return FakeBatch([image], [job.lambda_max()], self.log)
 
 
 def spec(seed, steps=100, points=1000, dt=1e-6, **kw):
@@ -239,9 +249,9 @@ def test_failures_end_one_request_only():
 
 def test_timeout_stops_a_request_and_its_batch():
     class Slow(FakeBatch):
-        def idle(self):
+        def device_step(self):
             time.sleep(2e-3)
-            return True
+            return self.step
 
     class SlowServer(FakeServer):
         def _batch(self, images, lambdas):
@@ -267,3 +277,26 @@ def test_batches_group_similar_step_counts():
         check_solo(r)
     batches = [e[1] for e in srv.log if e[0] == "batch"]
     assert batches[0] == [9] and sorted(batches[1]) == [0, 2, 4] and sorted(batches[2]) == [1, 3]
+
+
+def test_batch_wait_gathers_requests_that_share_a_batch():
+    # six requests of one key arriving 10 ms apart: admitted as they come
+    # (batch_wait = 0) the first runs alone; with a window they share one
+    # batch, and each still gets its solo results
+    arrivals = [(0.01 * i, spec(i, steps=40, power_polls=0)) for i in range(6)]
+    srv = FakeServer(max_slots=4, max_batch=8, chunk=10)
+    srv.serve(arrivals)
+    first = [e[1] for e in srv.log if e[0] == "batch"][0]
+    assert first == [0]
+    srv = FakeServer(max_slots=4, max_batch=8, chunk=10, batch_wait=0.5)
+    reqs = srv.serve(arrivals)
+    for r in reqs:
+        check_solo(r)
+    batches = [e[1] for e in srv.log if e[0] == "batch"]
+    assert batches == [[0, 1, 2, 3, 4, 5]]
+    # a full group does not wait
+    srv = FakeServer(max_slots=4, max_batch=3, chunk=10, batch_wait=30.0)
+    reqs = srv.serve(arrivals)
+    for r in reqs:
+        check_solo(r)
+    assert sorted(len(b) for b in (e[1] for e in srv.log if e[0] == "batch")) == [3, 3]
