@@ -19,8 +19,8 @@ sample (about 20 s) of the same workload, and (under "batched") the solves/s
 and latencies of 128 cased-hole dipole models run as batches against a
 sample of them run one after another on the plain path, and (under "served") the
 throughput and p50/p99 latency of a Poisson stream of mixed requests served
-with continuous batching at a sweep of arrival rates, against the same
-stream served one at a time.
+with continuous batching (at 8 requests/s by default; --serve-sweep adds
+other arrival rates), against the same stream served one at a time.
 
 `--calibrate-copy` instead runs a plain device-to-device copy of known size
 (the reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters).
@@ -136,24 +136,29 @@ def cpu_baseline(sim, dt, max_seconds=20.0):
 def measure_plain(solver, steps, warmup, what="the model"):
     """The plain path's figures of one model: `steps` replayed steps (the
     production path: step chunks replayed as hipGraphs) timed after
-    `warmup`, then up to 200 further steps launched directly with HIP events
-    around each force launch on the solver's stream, and the algorithmic
-    bytes of the step and of each block's force work."""
+    `warmup`, then 200 further steps launched directly with HIP events
+    around each force launch on the solver's stream (200 whatever `steps`
+    is, so that the per-launch average does not depend on it), and the
+    algorithmic bytes of the step and of each block's force work."""
     from insem_amd import _lib
 
     hip, check = _lib.hip(), _lib.check_hip
     ctx = ctypes.c_void_p(solver.context)
-    timing_steps = min(steps, 200)
+    timing_steps = 200
     need = warmup + steps + timing_steps
     if solver.n_steps < need:
         raise SystemExit(f"{what} has {solver.n_steps} steps, fewer than the {need} this run needs")
-    solver.advance(warmup)
-    # every step-chunk graph the timed advance replays is captured and
-    # instantiated here, outside the timed region
+    # every step-chunk graph the warmup and the timed advance replay is
+    # captured and instantiated here, before the warmup, so that the timed
+    # steps follow the warmup's without an idle device in between (a chunk's
+    # graph depends on its length and place in the advance, not on its step)
     t0 = time.perf_counter()
+    check(hip.insem_hip_prepare(ctx, warmup))
     check(hip.insem_hip_prepare(ctx, steps))
     check(hip.insem_hip_synchronize(ctx))
     capture_s = time.perf_counter() - t0
+    solver.advance(warmup)
+    check(hip.insem_hip_synchronize(ctx))
     t0 = time.perf_counter()
     solver.advance(steps)
     check(hip.insem_hip_synchronize(ctx))
@@ -354,10 +359,11 @@ def main():
     ap.add_argument("--batch-duration", type=float, default=5e-3, help="simulated time of each batched model (s)")
     ap.add_argument("--batch-size", type=int, default=5,
                     help="models per batch (each polynomial order's models, sorted by size, in batches of up to this)")
-    ap.add_argument("--serve-requests", type=int, default=100, help="requests of the served leg (0: skip it)")
+    ap.add_argument("--serve-requests", type=int, default=60, help="requests of the served leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=8.0,
                     help="Poisson arrival rate of the served leg's comparison with one-at-a-time service (1/s)")
-    ap.add_argument("--serve-sweep", default="2,4,8", help="arrival rates (1/s) the served loop runs at")
+    ap.add_argument("--serve-sweep", default="8",
+                    help="arrival rates (1/s) the served loop runs at, comma-separated (e.g. 2,4,8)")
     ap.add_argument("--serve-slots", type=int, default=4, help="batches the served loop keeps on the device at once")
     ap.add_argument("--serve-batch", type=int, default=8, help="requests per batch of the served loop")
     args = ap.parse_args()

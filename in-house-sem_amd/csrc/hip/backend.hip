@@ -75,6 +75,90 @@ struct HipFailure {
     }                                                                                                  \
   } while (0)
 
+// Device blocks of destroyed contexts are kept per device and handed to
+// later contexts instead of going back through hipFree, which waits for
+// every stream of the device: a context destroyed while other contexts have
+// steps queued (a finished batch, a power iteration's context) would stall
+// its thread until all of them finish.  A cached block serves a request of
+// at least 2/3 of its size; a device's cache holds at most kCacheCap bytes
+// (the largest blocks go back to the device beyond that) and is emptied
+// when an allocation fails for want of memory.  Guarded by g_capture_mu.
+constexpr int kCacheDevices = 64;
+constexpr size_t kCacheCap = size_t{32} << 30;
+
+struct BlockCache {
+  std::multimap<size_t, void*> blocks;  // size -> block
+  size_t bytes = 0;
+};
+BlockCache g_cache[kCacheDevices];
+
+// The device's cache (devices beyond kCacheDevices allocate and free directly).
+BlockCache* cache_of(int device) { return device >= 0 && device < kCacheDevices ? &g_cache[device] : nullptr; }
+
+// Allocation granularity: 4 KiB up to 1 MiB, 2 MiB above.
+size_t block_bytes(size_t bytes) {
+  const size_t g = bytes <= (size_t{1} << 20) ? size_t{4} << 10 : size_t{2} << 20;
+  return (bytes + g - 1) / g * g;
+}
+
+// Frees every cached block of the device (waits for the device's streams).
+// Caller holds g_capture_mu.
+void release_cache_locked(int device) {
+  BlockCache* bc = cache_of(device);
+  if (!bc) return;
+  for (auto& b : bc->blocks) (void)hipFree(b.second);
+  bc->blocks.clear();
+  bc->bytes = 0;
+}
+
+// A block of at least `bytes` on the current device (`device`); its size in
+// *size.  *reused tells whether it held another context's data.
+void* take_block(int device, size_t bytes, size_t* size, bool* reused) {
+  std::lock_guard<std::mutex> lock(g_capture_mu);
+  BlockCache* bc = cache_of(device);
+  const size_t want = block_bytes(bytes);
+  if (bc) {
+    auto it = bc->blocks.lower_bound(want);
+    if (it != bc->blocks.end() && it->first <= want + want / 2) {
+      void* p = it->second;
+      *size = it->first;
+      bc->bytes -= it->first;
+      bc->blocks.erase(it);
+      *reused = true;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e == hipErrorOutOfMemory && bc && !bc->blocks.empty()) {
+    (void)hipGetLastError();
+    release_cache_locked(device);
+    e = hipMalloc(&p, want);
+  }
+  HIP_TRY(e);
+  *size = want;
+  *reused = false;
+  return p;
+}
+
+// Returns a block whose last use on the device has finished.
+void give_block(int device, void* p, size_t size) {
+  std::lock_guard<std::mutex> lock(g_capture_mu);
+  BlockCache* bc = cache_of(device);
+  if (!bc || size > kCacheCap) {
+    (void)hipFree(p);
+    return;
+  }
+  while (bc->bytes + size > kCacheCap && !bc->blocks.empty()) {
+    auto last = std::prev(bc->blocks.end());
+    (void)hipFree(last->second);
+    bc->bytes -= last->first;
+    bc->blocks.erase(last);
+  }
+  bc->blocks.emplace(size, p);
+  bc->bytes += size;
+}
+
 struct BlockRT {
   int phys = 0;
   bool rect = false;
@@ -128,7 +212,7 @@ struct TraceCol {
 struct insem_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  std::vector<void*> allocs;
+  std::vector<std::pair<void*, size_t>> allocs;  // device blocks and their sizes
   int64_t footprint = 0;
   int ngll = 0;
   float *D = nullptr, *Dj = nullptr, *w2 = nullptr;
@@ -194,16 +278,18 @@ struct insem_hip_ctx {
   double force_ms = 0.0, update_ms = 0.0;
   int force_n = 0, update_n = 0;
 
+  // n values on the context's device; a block that held another
+  // context's data is cleared first unless `clear` is false (the caller
+  // writes all n values before any kernel reads them).
   template <class T>
-  T* alloc(size_t n) {
+  T* alloc(size_t n, bool clear = true) {
     if (n == 0) n = 1;
-    void* p = nullptr;
-    {
-      std::lock_guard<std::mutex> lock(g_capture_mu);
-      HIP_TRY(hipMalloc(&p, n * sizeof(T)));
-    }
-    allocs.push_back(p);
+    size_t size = 0;
+    bool reused = false;
+    void* p = take_block(device, n * sizeof(T), &size, &reused);
+    allocs.emplace_back(p, size);
     footprint += static_cast<int64_t>(n * sizeof(T));
+    if (reused && clear) zero(p, size);
     return static_cast<T*>(p);
   }
   // Copies and fills go through the context's own stream and wait for it:
@@ -223,7 +309,7 @@ struct insem_hip_ctx {
   void zero(void* d, size_t bytes) { HIP_TRY(hipMemsetAsync(d, 0, bytes, stream)); }
   template <class T>
   T* upload(const T* src, size_t n) {
-    T* d = alloc<T>(n);
+    T* d = alloc<T>(n, n == 0);
     if (n) h2d(d, src, n * sizeof(T));
     return d;
   }
@@ -298,10 +384,8 @@ void free_all(insem_hip_ctx* c) {
   c->graphs.clear();
   for (hipGraphExec_t g : c->lam_execs) (void)hipGraphExecDestroy(g);
   c->lam_execs.clear();
-  {
-    std::lock_guard<std::mutex> lock(g_capture_mu);
-    for (void* p : c->allocs) (void)hipFree(p);
-  }
+  // the context's stream is idle: its blocks can serve other contexts
+  for (auto& b : c->allocs) give_block(c->device, b.first, b.second);
   c->allocs.clear();
   for (auto& e : c->ev_force) {
     (void)hipEventDestroy(e.first);
@@ -1808,10 +1892,10 @@ int insem_hip_block_forces(insem_hip_ctx* c, int32_t di, int32_t bi, const float
     HIP_TRY(launch_forces(c->ngll, br.phys, br.rect, false, a, StepParams(), c->stream));
     HIP_TRY(hipMemcpyAsync(fl, dout, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    const auto blk = c->allocs.back();
     c->allocs.pop_back();
     c->footprint -= static_cast<int64_t>(n * sizeof(float));
-    std::lock_guard<std::mutex> lock(g_capture_mu);
-    HIP_TRY(hipFree(dout));
+    give_block(c->device, blk.first, blk.second);
   });
 }
 
@@ -2016,8 +2100,24 @@ int insem_hip_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
   size_t f = 0, t = 0;
   const hipError_t e = hipMemGetInfo(&f, &t);
   if (e != hipSuccess) return set_error(INSEM_HIP_EDEVICE, "hipMemGetInfo failed: %s", hipGetErrorString(e));
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    std::lock_guard<std::mutex> lock(g_capture_mu);
+    if (const BlockCache* bc = cache_of(dev)) f += bc->bytes;  // cached blocks serve new contexts
+  }
   *free_bytes = static_cast<int64_t>(f);
   *total_bytes = static_cast<int64_t>(t);
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_release_cache(int64_t* bytes) {
+  int dev = 0;
+  const hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error(INSEM_HIP_EDEVICE, "hipGetDevice failed: %s", hipGetErrorString(e));
+  std::lock_guard<std::mutex> lock(g_capture_mu);
+  const BlockCache* bc = cache_of(dev);
+  if (bytes) *bytes = bc ? static_cast<int64_t>(bc->bytes) : 0;
+  release_cache_locked(dev);
   return INSEM_HIP_OK;
 }
 

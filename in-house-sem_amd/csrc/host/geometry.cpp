@@ -73,6 +73,18 @@ class KindVote {
   double largest_ = 0.0, spread_ = 0.0, shear_ = 0.0;
 };
 
+// The bounding box of leaf e's points.
+std::array<double, 4> point_box(const Geometry& g, int32_t e) {
+  const std::size_t npe = static_cast<std::size_t>(g.ngll) * g.ngll;
+  const std::size_t p0 = static_cast<std::size_t>(e) * npe;
+  double lo_x = g.x[p0], hi_x = lo_x, lo_z = g.z[p0], hi_z = lo_z;
+  for (std::size_t p = p0; p < p0 + npe; ++p) {
+    lo_x = std::min(lo_x, g.x[p]), hi_x = std::max(hi_x, g.x[p]);
+    lo_z = std::min(lo_z, g.z[p]), hi_z = std::max(hi_z, g.z[p]);
+  }
+  return {lo_x, hi_x, lo_z, hi_z};
+}
+
 }  // namespace
 
 std::array<double, 2> leaf_map(const Forest& f, int32_t leaf, double xi, double eta) {
@@ -94,6 +106,7 @@ Geometry compute_geometry(const Forest& f, const std::vector<double>& ref, const
   const std::size_t npe = static_cast<std::size_t>(ng) * ng;
   for (auto* v : {&g.x, &g.z, &g.xi_x, &g.xi_z, &g.eta_x, &g.eta_z, &g.jac}) v->assign(npe * g.n_elements, 0.0);
   g.kind.assign(static_cast<std::size_t>(g.n_elements), Kind::Rect);
+  g.box.assign(static_cast<std::size_t>(g.n_elements), {0.0, 0.0, 0.0, 0.0});
   int32_t bad_leaf = -1;
 #pragma omp parallel for schedule(static)
   for (int32_t e = 0; e < g.n_elements; ++e) {
@@ -121,6 +134,7 @@ Geometry compute_geometry(const Forest& f, const std::vector<double>& ref, const
         }
       }
     g.kind[e] = vote.kind();
+    g.box[e] = point_box(g, e);
   }
   INSEM_REQUIRE(bad_leaf < 0, "leaf " << bad_leaf << " maps onto the plane with a non-positive Jacobian");
   return g;
@@ -128,15 +142,11 @@ Geometry compute_geometry(const Forest& f, const std::vector<double>& ref, const
 
 std::optional<Location> locate(const Forest& f, const Geometry& g, std::array<double, 2> x,
                                const std::vector<int32_t>& candidates) {
-  const std::size_t npe = static_cast<std::size_t>(g.ngll) * g.ngll;
+  const bool boxed = g.box.size() == static_cast<std::size_t>(g.n_elements);
   for (int32_t leaf : candidates) {
     // cheap rejection: the leaf's points' bounding box, widened by a quarter
-    const std::size_t p0 = static_cast<std::size_t>(leaf) * npe;
-    double lo_x = g.x[p0], hi_x = lo_x, lo_z = g.z[p0], hi_z = lo_z;
-    for (std::size_t p = p0; p < p0 + npe; ++p) {
-      lo_x = std::min(lo_x, g.x[p]), hi_x = std::max(hi_x, g.x[p]);
-      lo_z = std::min(lo_z, g.z[p]), hi_z = std::max(hi_z, g.z[p]);
-    }
+    const std::array<double, 4> b = boxed ? g.box[leaf] : point_box(g, leaf);
+    const double lo_x = b[0], hi_x = b[1], lo_z = b[2], hi_z = b[3];
     const double margin = 0.25 * std::max(hi_x - lo_x, hi_z - lo_z);
     if (x[0] < lo_x - margin || x[0] > hi_x + margin || x[1] < lo_z - margin || x[1] > hi_z + margin) continue;
     // Newton on map(xi, eta) = x from the leaf's centre, iterates kept in

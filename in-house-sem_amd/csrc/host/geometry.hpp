@@ -27,6 +27,9 @@ struct Geometry {
   int32_t n_elements = 0;
   std::vector<double> x, z, xi_x, xi_z, eta_x, eta_z, jac;  // [leaf][k]
   std::vector<Kind> kind;                                    // per leaf
+  // per leaf: the bounding box (min x, max x, min z, max z) of its points
+  // (locate's cheap rejection; filled by compute_geometry)
+  std::vector<std::array<double, 4>> box;
 };
 
 std::array<double, 2> leaf_map(const Forest& f, int32_t leaf, double xi, double eta);

@@ -52,6 +52,36 @@ class PointSums {
   std::vector<int32_t> touched_;
 };
 
+// The same sums for the few points one source or receiver touches (tens,
+// where the medium has millions): a short list searched from its end, so
+// that no medium-sized array is cleared per source or receiver.
+template <int W>
+class FewPointSums {
+ public:
+  double* at(int32_t g) {
+    for (std::size_t k = points_.size(); k-- > 0;)
+      if (points_[k] == g) return sums_[k].data();
+    points_.push_back(g);
+    sums_.push_back({});
+    return sums_.back().data();
+  }
+  const double* get(int32_t g) const {
+    std::size_t k = 0;
+    while (k < points_.size() && points_[k] != g) ++k;
+    INSEM_REQUIRE(k < points_.size(), "point " << g << " has no sum");
+    return sums_[k].data();
+  }
+  std::vector<int32_t> ascending() const {
+    std::vector<int32_t> out = points_;
+    std::sort(out.begin(), out.end());
+    return out;
+  }
+
+ private:
+  std::vector<int32_t> points_;
+  std::vector<std::array<double, W>> sums_;
+};
+
 class ImageBuilder {
  public:
   ImageBuilder(const Forest& F, const Geometry& G, const Rule& gll, int azimuthal_order,
@@ -503,7 +533,7 @@ class ImageBuilder {
     const double rho = material(loc->leaf).rho;
     const bool split_radial = d.medium == Medium::Elastic && d.azimuthal_order == 1;
     const int32_t e = d.element_of_leaf[loc->leaf];
-    PointSums<3> coef(d.dofs.n_global);
+    FewPointSums<3> coef;
     for (std::size_t k = 0; k < npe_; ++k) {
       double fc[3] = {0.0, 0.0, 0.0};
       switch (src.type) {
@@ -564,7 +594,7 @@ class ImageBuilder {
     const bool axial = axisym_ && F_.is_axial(loc->leaf);
     const PointWeights w = point_weights(F_, axial ? lag_glj_ : lag_, lag_, *loc);
     const int32_t e = d.element_of_leaf[loc->leaf];
-    PointSums<3> sums(d.dofs.n_global);
+    FewPointSums<3> sums;
     for (std::size_t k = 0; k < npe_; ++k)
       for (const auto& [g, wt] : d.dofs.slot_entries(e, static_cast<int>(k))) {
         double* s = sums.at(g);

@@ -75,6 +75,7 @@ def hip():
         _declare(lib, "insem_hip_synchronize", c_int, c_void)
         _declare(lib, "insem_hip_query", c_int, c_void, P(c_int))
         _declare(lib, "insem_hip_device_memory", c_int, P(c_i64), P(c_i64))
+        _declare(lib, "insem_hip_release_cache", c_int, P(c_i64))
         _declare(lib, "insem_hip_set_timing", c_int, c_void, c_int)
         _declare(lib, "insem_hip_kernel_time", c_int, c_void, P(c_double), P(c_int), P(c_double), P(c_int))
         _declare(lib, "insem_hip_block_time", c_int, c_void, c_int, c_int, P(c_double), P(c_int))

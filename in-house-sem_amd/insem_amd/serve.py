@@ -232,6 +232,19 @@ class Server:
     def _solo(self, job, image: SetupImage):
         return _Solo(job.into_solver(image))
 
+    def _start_power(self, image: SetupImage) -> Future:
+        """The power iteration's start (its context's set-up: host work and
+        uploads), on the loop thread: device calls stay on the thread that
+        makes every other device call of the loop (contexts set up on
+        worker threads while the loop replays graphs have stalled)."""
+        f: Future = Future()
+        try:
+            with self._timed("power_start_s"):
+                f.set_result(self._power_job(image))
+        except Exception as e:  # noqa: BLE001 - reported through the future
+            f.set_exception(e)
+        return f
+
     def _prepare(self, r: Request) -> Request:
         r.image = self._setup(r)
         r.key = batch_key(r.image)
@@ -350,8 +363,8 @@ class Server:
         """Runs the stream (arrival times in s from now) to completion; returns
         the requests in arrival order with their timings and results."""
         reqs = [Request(i, sim, float(t)) for i, (t, sim) in enumerate(sorted(arrivals, key=lambda a: a[0]))]
-        self.stats = {"iterations": 0, "power_start_s": 0.0, "batch_setup_s": 0.0, "advance_s": 0.0,
-                      "deliver_s": 0.0}
+        self.stats = {"iterations": 0, "power_start_s": 0.0, "power_result_s": 0.0, "batch_setup_s": 0.0,
+                      "advance_s": 0.0, "deliver_s": 0.0, "close_s": 0.0}
         self.events = []  # (s since start, "admit", request ids, their step counts)
         pending = deque(reqs)
         building: deque = deque()   # (request, future of its setup) in arrival order
@@ -387,21 +400,27 @@ class Server:
                         r.image = None
                         continue
                     timing.append(r)
-                # 2. power iterations: started within the budget, polled without waiting
+                # 2. power iterations: started within the budget, polled
+                # without waiting
                 while timing and len(power) < self.max_power_jobs and used + timing[0].footprint <= self.hbm_budget:
                     r = timing.popleft()
+                    power.append((r, self._start_power(r.image)))
+                    used += r.footprint
+                    busy = True
+                for r, started in list(power):
+                    if not started.done():
+                        continue
                     try:
-                        with self._timed("power_start_s"):
-                            power.append((r, self._power_job(r.image)))
-                        used += r.footprint
-                    except Exception as e:  # noqa: BLE001
+                        job = started.result()
+                    except Exception as e:  # noqa: BLE001 - the request fails, the loop goes on
+                        power.remove((r, started))
+                        used -= r.footprint
                         self._fail(r, f"power iteration failed: {e}", t0)
                         r.image = None
-                    busy = True
-                for r, job in list(power):
+                        continue
                     if not job.ready():
                         continue
-                    power.remove((r, job))
+                    power.remove((r, started))
                     if r.large:
                         # the job's context becomes the request's solver: its
                         # bytes stay counted until the slot closes
@@ -412,7 +431,8 @@ class Server:
                         continue
                     used -= r.footprint
                     try:
-                        r.lam = job.lambda_max()
+                        with self._timed("power_result_s"):
+                            r.lam = job.lambda_max()
                         r.dt, r.n_steps = r.image.time_step(r.lam)
                     except Exception as e:  # noqa: BLE001
                         self._fail(r, f"power iteration failed: {e}", t0)
@@ -464,7 +484,8 @@ class Server:
                         slot.open.remove(m)
                         self._fail(slot.requests[m], f"timed out after {self.timeout:g} s", t0)
                     if not slot.open:
-                        b.close()  # waits for steps still queued (of finished or expired models)
+                        with self._timed("close_s"):
+                            b.close()  # waits for steps still queued (of finished or expired models)
                         slots.remove(slot)
                         used -= slot.footprint
                         continue

@@ -9,9 +9,13 @@
  * (reference: cpp/include/insem/core/error.hpp:9-25), Python as InsemError.
  *
  * Data handed to insem_hip_create() is copied; the caller keeps ownership of
- * every host buffer.  The context owns all device memory and frees it in
- * insem_hip_destroy().  A context is not thread-safe (one host thread per
- * context, as the reference's Solver).
+ * every host buffer.  The context owns all device memory and releases it in
+ * insem_hip_destroy(): its blocks go to a per-device cache that later
+ * contexts draw from (hipFree would wait for every stream of the device,
+ * stalling a host that destroys one context while others have work queued);
+ * insem_hip_release_cache() hands the cache back to the device.  A context
+ * is not thread-safe (one host thread per context, as the reference's
+ * Solver); contexts may be created and destroyed on several threads.
  *
  * Layouts follow the reference exactly (cpp/include/insem/ops/dof_map.hpp:15-18):
  *   global field:  [component][global point]
@@ -322,8 +326,13 @@ int insem_hip_synchronize(insem_hip_ctx* ctx);
 /* *idle = 1 when all queued work of the context has finished, else 0; does
  * not block (a request loop polls it to keep several contexts busy). */
 int insem_hip_query(insem_hip_ctx* ctx, int32_t* idle);
-/* Free and total bytes of the current device (admission control). */
+/* Free and total bytes of the current device (admission control); free
+ * bytes include the blocks cached for new contexts. */
 int insem_hip_device_memory(int64_t* free_bytes, int64_t* total_bytes);
+/* Frees the current device's cached blocks (those of destroyed contexts),
+ * waiting for the device's streams; *bytes = bytes released (may be NULL).
+ * No counterpart in the reference, whose buffers are host vectors. */
+int insem_hip_release_cache(int64_t* bytes);
 
 /*
  * Kernel timing with HIP events on the context's stream.  When enabled,

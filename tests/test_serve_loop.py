@@ -6,6 +6,7 @@ by the GPU tests of the served mix."""
 
 from __future__ import annotations
 
+import threading
 import time
 from types import SimpleNamespace
 
@@ -114,6 +115,7 @@ class FakeServer(serve.Server):
     def _power_job(self, img):
         self.log.append(("power", img.spec["seed"]))
         return FakeJob(img, self.log)
+
 
     def _batch(self, images, lambdas):
         if any(im.spec.get("batch_fails") for im in images):
@@ -300,3 +302,25 @@ def test_batch_wait_gathers_requests_that_share_a_batch():
     for r in reqs:
         check_solo(r)
     assert sorted(len(b) for b in (e[1] for e in srv.log if e[0] == "batch")) == [3, 3]
+
+
+
+def test_power_iterations_start_on_the_loop_thread():
+    # every device call of the loop, a power iteration's start included, is
+    # made on the loop's own thread; a start that fails ends its request only
+
+    class Starts(FakeServer):
+        threads = set()
+
+        def _power_job(self, img):
+            Starts.threads.add(threading.get_ident())
+            if img.spec.get("start_fails"):
+                raise RuntimeError("no device memory")
+            return super()._power_job(img)
+
+    srv = Starts(max_slots=2, max_batch=2, chunk=50)
+    reqs = srv.serve([(0.0, spec(0)), (0.0, spec(1, start_fails=True)), (0.0, spec(2)), (0.0, spec(3))])
+    assert Starts.threads == {threading.get_ident()}
+    for r in (reqs[0], reqs[2], reqs[3]):
+        check_solo(r)
+    assert reqs[1].error.startswith("power iteration failed")
