@@ -1148,6 +1148,11 @@ void initialize(insem_hip_ctx* c) {
 
 void predictor_all(insem_hip_ctx* c) {
   const StepCoef sc = step_coef(c, true);
+  if (!c->n_models && c->doms.size() <= 2) {
+    HIP_TRY(launch_predictor_flat(c->doms.empty() ? nullptr : &c->doms[0].dd,
+                                  c->doms.size() > 1 ? &c->doms[1].dd : nullptr, sc, c->stream));
+    return;
+  }
   for (DomainRT& R : c->doms) HIP_TRY(launch_predictor(R.dd, sc, clock_at(c, 0), c->stream));
 }
 

@@ -107,6 +107,9 @@ hipError_t launch_clock_add(int32_t* base, int n, hipStream_t s);
 // u += rdt v + hdt2 a; v += hdt a over all points, for the step after the
 // clock's (per-model coefficients on a batch).
 hipError_t launch_predictor(const DomDev& d, const StepCoef& sc, const StepClock& clk, hipStream_t s);
+// The predictor of one model (not a batch) over both domains in one launch
+// (d1 may be null).
+hipError_t launch_predictor_flat(const DomDev* d0, const DomDev* d1, const StepCoef& sc, hipStream_t s);
 // f = assembly of f_local over all points (no update).
 hipError_t launch_assemble(const DomDev& d, hipStream_t s);
 // Fixed-order sum of n rows of `width` doubles into out (+= when accumulate).

@@ -508,6 +508,50 @@ __global__ __launch_bounds__(kTB) void k_predictor(DomDev d, StepCoef sc, StepCl
   d.v[i] = vi + ps.hdt * ai;
 }
 
+// The same predictor over every value of up to two domains of one model
+// (no per-point coefficients): one work-item per four consecutive values of
+// a domain's u, v and a (component-major, 16-byte aligned), so the launch
+// streams the three fields at full width.  Each value gets the operations
+// of k_predictor in the same order.
+struct PredSpan {
+  float* u = nullptr;
+  float* v = nullptr;
+  const float* a = nullptr;
+  int64_t n = 0;  // values
+};
+
+__global__ __launch_bounds__(kTB) void k_predictor_flat(PredSpan s0, PredSpan s1, int64_t q0, int64_t q1, float rdt,
+                                                       float hdt, float hdt2) {
+  int64_t q = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
+  if (q >= q0 + q1) return;
+  const bool second = q >= q0;
+  const PredSpan& sp = second ? s1 : s0;
+  if (second) q -= q0;
+  const int64_t i = 4 * q;
+  if (i + 4 <= sp.n) {
+    const float4 u = *reinterpret_cast<const float4*>(sp.u + i);
+    const float4 v = *reinterpret_cast<const float4*>(sp.v + i);
+    const float4 a = *reinterpret_cast<const float4*>(sp.a + i);
+    float4 un, vn;
+    un.x = u.x + (rdt * v.x + hdt2 * a.x);
+    un.y = u.y + (rdt * v.y + hdt2 * a.y);
+    un.z = u.z + (rdt * v.z + hdt2 * a.z);
+    un.w = u.w + (rdt * v.w + hdt2 * a.w);
+    vn.x = v.x + hdt * a.x;
+    vn.y = v.y + hdt * a.y;
+    vn.z = v.z + hdt * a.z;
+    vn.w = v.w + hdt * a.w;
+    *reinterpret_cast<float4*>(sp.u + i) = un;
+    *reinterpret_cast<float4*>(sp.v + i) = vn;
+  } else {
+    for (int64_t k = i; k < sp.n; ++k) {
+      const float vi = sp.v[k], ai = sp.a[k];
+      sp.u[k] = sp.u[k] + (rdt * vi + hdt2 * ai);
+      sp.v[k] = vi + hdt * ai;
+    }
+  }
+}
+
 template <int C, bool W>
 __global__ __launch_bounds__(kTB) void k_assemble(DomDev d) {
   const int64_t g = static_cast<int64_t>(blockIdx.x) * kTB + threadIdx.x;
@@ -753,6 +797,25 @@ hipError_t launch_predictor(const DomDev& d, const StepCoef& sc, const StepClock
   const int64_t n = d.ncomp * d.ng;
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_predictor, dim3(blocks_for(n)), dim3(kTB), 0, s, d, sc, clk);
+  return hipGetLastError();
+}
+
+hipError_t launch_predictor_flat(const DomDev* d0, const DomDev* d1, const StepCoef& sc, hipStream_t s) {
+  PredSpan sp[2];
+  int64_t q[2] = {0, 0};
+  const DomDev* dd[2] = {d0, d1};
+  for (int k = 0; k < 2; ++k) {
+    if (!dd[k]) continue;
+    const DomDev& d = *dd[k];
+    if ((reinterpret_cast<uintptr_t>(d.u) | reinterpret_cast<uintptr_t>(d.v) | reinterpret_cast<uintptr_t>(d.a)) & 15)
+      return hipErrorInvalidValue;  // fields are 256-byte aligned allocations
+    sp[k] = PredSpan{d.u, d.v, d.a, d.ncomp * d.ng};
+    q[k] = (sp[k].n + 3) / 4;
+  }
+  const int64_t nb = (q[0] + q[1] + kTB - 1) / kTB;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_predictor_flat, dim3(static_cast<unsigned>(nb)), dim3(kTB), 0, s, sp[0], sp[1], q[0], q[1],
+                     sc.rdt, sc.hdt, sc.hdt2);
   return hipGetLastError();
 }
 
