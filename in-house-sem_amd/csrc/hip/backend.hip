@@ -1241,20 +1241,23 @@ int guarded(insem_hip_ctx* c, F&& f) {
   }
 }
 
-// Queues n repetitions of the launches of f: f captured once as a hipGraph,
-// the graph launched n times (instantiation costs grow with the node
-// count), or f called n times when graphs are off.
+// Queues n repetitions of the launches of f: f captured r times as one
+// hipGraph (r the largest divisor of n up to 10: fewer graph launches on the
+// host, a graph small enough to instantiate quickly), the graph launched
+// n / r times, or f called n times when graphs are off.
 template <class F>
 void queue_repeated(insem_hip_ctx* c, int n, F&& f) {
   if (!c->use_graphs) {
     for (int k = 0; k < n; ++k) f();
     return;
   }
+  int r = std::min(n, 10);
+  while (r > 1 && n % r != 0) --r;
   std::unique_lock<std::mutex> lock(g_capture_mu);
   HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   hipGraph_t g = nullptr;
   try {
-    f();
+    for (int k = 0; k < r; ++k) f();
   } catch (...) {
     (void)hipStreamEndCapture(c->stream, &g);
     if (g) (void)hipGraphDestroy(g);
@@ -1267,7 +1270,7 @@ void queue_repeated(insem_hip_ctx* c, int n, F&& f) {
   (void)hipGraphDestroy(g);
   HIP_TRY(e);
   c->lam_execs.push_back(ge);
-  for (int k = 0; k < n; ++k) HIP_TRY(hipGraphLaunch(ge, c->stream));
+  for (int k = 0; k < n / r; ++k) HIP_TRY(hipGraphLaunch(ge, c->stream));
 }
 
 void permute_in(const DomainRT& R, const float* src, std::vector<float>& out) {

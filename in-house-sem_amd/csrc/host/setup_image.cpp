@@ -4,6 +4,7 @@
 // mass, boundary terms, the fluid-solid coupling, sources, receivers.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <random>
 
 #include "error.hpp"
@@ -738,20 +739,42 @@ void SetupImage::refresh_views() {
 // 629-630): one std::mt19937 seeded 12345 and U(-1, 1) doubles, rounded to
 // float; each domain in domain order, then for coupled models the solid's
 // and the fluid's again.
-std::vector<float> SetupImage::power_start_vectors() const {
-  std::mt19937 gen(12345);
-  std::uniform_real_distribution<double> u(-1.0, 1.0);
-  std::vector<float> out;
-  auto draw = [&](const Domain& d) {
-    const std::size_t len = static_cast<std::size_t>(d.ncomp) * d.dofs.n_global;
-    for (std::size_t i = 0; i < len; ++i) out.push_back(static_cast<float>(u(gen)));
-  };
-  for (const Domain& d : domains) draw(d);
-  if (!coupling.fluid_points.empty()) {
-    draw(domains[domain_index(Medium::Elastic)]);
-    draw(domains[domain_index(Medium::Acoustic)]);
+namespace {
+
+// The start vectors of every model are a prefix of one stream (one
+// generator seeded 12345, drawn in order), kept once per process and
+// extended as longer models need it.
+class StartStream {
+ public:
+  std::vector<float> prefix(std::size_t n) {
+    std::lock_guard<std::mutex> lock(mu_);
+    while (values_.size() < n) values_.push_back(static_cast<float>(u_(gen_)));
+    return std::vector<float>(values_.begin(), values_.begin() + static_cast<std::ptrdiff_t>(n));
   }
-  return out;
+
+ private:
+  std::mutex mu_;
+  std::mt19937 gen_{12345};
+  std::uniform_real_distribution<double> u_{-1.0, 1.0};
+  std::vector<float> values_;
+};
+
+StartStream& start_stream() {
+  static StartStream s;
+  return s;
+}
+
+}  // namespace
+
+std::vector<float> SetupImage::power_start_vectors() const {
+  std::size_t n = 0;
+  for (const Domain& d : domains) n += static_cast<std::size_t>(d.ncomp) * d.dofs.n_global;
+  if (!coupling.fluid_points.empty()) {
+    const Domain& s = domains[domain_index(Medium::Elastic)];
+    const Domain& f = domains[domain_index(Medium::Acoustic)];
+    n += static_cast<std::size_t>(s.ncomp) * s.dofs.n_global + static_cast<std::size_t>(f.ncomp) * f.dofs.n_global;
+  }
+  return start_stream().prefix(n);
 }
 
 std::vector<float> SetupImage::source_amplitudes(double dt, int n_steps) const {
