@@ -47,6 +47,14 @@ struct MeshHandle {
   std::shared_ptr<const MacroMesh> mesh;
 };
 
+// The object behind a handle; a null handle (a closed object, or an image a
+// batch or solver took over) is an error, not a crash.
+template <class T>
+T* ref(void* h) {
+  INSEM_REQUIRE(h, "null handle: the object was closed or taken over");
+  return static_cast<T*>(h);
+}
+
 std::vector<std::string> split_csv(const char* s) {
   std::vector<std::string> out;
   std::stringstream ss(s ? s : "");
@@ -98,7 +106,7 @@ void insem_mesh_free(void* h) { delete static_cast<MeshHandle*>(h); }
 
 int insem_mesh_info(void* h, int32_t* out) {
   return guard([&] {
-    const MacroMesh& m = *static_cast<MeshHandle*>(h)->mesh;
+    const MacroMesh& m = *ref<MeshHandle>(h)->mesh;
     out[0] = m.n_nodes();
     out[1] = m.n_cells();
     out[2] = m.n_edges();
@@ -110,7 +118,7 @@ int insem_mesh_info(void* h, int32_t* out) {
 
 int insem_mesh_arrays(void* h, int32_t* cells, int32_t* boundary) {
   return guard([&] {
-    const MacroMesh& m = *static_cast<MeshHandle*>(h)->mesh;
+    const MacroMesh& m = *ref<MeshHandle>(h)->mesh;
     if (cells) std::memcpy(cells, m.cells().data(), m.cells().size() * sizeof(int32_t));
     if (boundary)
       for (size_t i = 0; i < m.boundary().size(); ++i) {
@@ -123,7 +131,7 @@ int insem_mesh_arrays(void* h, int32_t* cells, int32_t* boundary) {
 
 int insem_mesh_map(void* h, int32_t cell, double xi, double eta, double* out) {
   return guard([&] {
-    const MacroMesh& m = *static_cast<MeshHandle*>(h)->mesh;
+    const MacroMesh& m = *ref<MeshHandle>(h)->mesh;
     INSEM_REQUIRE(cell >= 0 && cell < m.n_cells(), "cell " << cell << " out of range");
     const auto x = m.map(cell, xi, eta);
     out[0] = x[0];
@@ -132,8 +140,9 @@ int insem_mesh_map(void* h, int32_t cell, double xi, double eta, double* out) {
 }
 
 int insem_mesh_is_axis_face(void* h, int32_t cell, int32_t face) {
-  const MacroMesh& m = *static_cast<MeshHandle*>(h)->mesh;
-  return m.is_axis_face(cell, face) ? 1 : 0;
+  int out = -1;
+  (void)guard([&] { out = ref<MeshHandle>(h)->mesh->is_axis_face(cell, face) ? 1 : 0; });
+  return out;
 }
 
 // ----------------------------------------------------------------- model
@@ -164,14 +173,14 @@ int insem_model_set_material(void* h, int32_t region, int32_t medium, double vp,
     m.vp = vp;
     m.vs = medium == 0 ? 0.0 : vs;
     m.rho = rho;
-    static_cast<Model*>(h)->set_material(region, m);
+    ref<Model>(h)->set_material(region, m);
   });
 }
 
 int insem_model_set_boundary(void* h, int32_t tag, int32_t type) {
   return guard([&] {
     INSEM_REQUIRE(type >= 0 && type <= 2, "unknown boundary type " << type);
-    static_cast<Model*>(h)->set_boundary(tag, static_cast<Boundary>(type));
+    ref<Model>(h)->set_boundary(tag, static_cast<Boundary>(type));
   });
 }
 
@@ -194,12 +203,12 @@ int insem_model_add_source(void* h, int32_t type, double x, double z, double c0,
     } else {
       s.stf = Stf::make(stf_type, f0, t0, amp);
     }
-    static_cast<Model*>(h)->add_source(s);
+    ref<Model>(h)->add_source(s);
   });
 }
 
 int insem_model_clear_sources(void* h) {
-  return guard([&] { static_cast<Model*>(h)->clear_sources(); });
+  return guard([&] { ref<Model>(h)->clear_sources(); });
 }
 
 int insem_model_add_receiver(void* h, const char* name, double x, double z, const char* quantities, const char* medium) {
@@ -209,12 +218,12 @@ int insem_model_add_receiver(void* h, const char* name, double x, double z, cons
     r.position = {x, z};
     r.quantities = split_csv(quantities);
     r.medium = medium ? medium : "";
-    static_cast<Model*>(h)->add_receiver(r);
+    ref<Model>(h)->add_receiver(r);
   });
 }
 
 int insem_model_clear_receivers(void* h) {
-  return guard([&] { static_cast<Model*>(h)->clear_receivers(); });
+  return guard([&] { ref<Model>(h)->clear_receivers(); });
 }
 
 int insem_model_set_time(void* h, double dt, double safety, int32_t n_steps, double duration, int32_t record_every,
@@ -227,17 +236,17 @@ int insem_model_set_time(void* h, double dt, double safety, int32_t n_steps, dou
     t.duration = duration;
     t.record_every = record_every;
     t.energy_every = energy_every;
-    static_cast<Model*>(h)->set_time(t);
+    ref<Model>(h)->set_time(t);
   });
 }
 
 int insem_model_setup(void* h) {
-  return guard([&] { static_cast<Model*>(h)->setup(); });
+  return guard([&] { ref<Model>(h)->setup(); });
 }
 
 int insem_model_info(void* h, int32_t* out) {
   return guard([&] {
-    const Model& m = *static_cast<Model*>(h);
+    const Model& m = *ref<Model>(h);
     out[0] = m.forest().n_leaves();
     out[1] = m.forest().max_level();
     const auto c = m.kind_counts();
@@ -249,25 +258,27 @@ int insem_model_info(void* h, int32_t* out) {
 
 int insem_model_leaf_points(void* h, double* x, double* z) {
   return guard([&] {
-    const Geometry& g = static_cast<Model*>(h)->geometry();
+    const Geometry& g = ref<Model>(h)->geometry();
     std::memcpy(x, g.x.data(), g.x.size() * sizeof(double));
     std::memcpy(z, g.z.data(), g.z.size() * sizeof(double));
   });
 }
 
 void* insem_model_build(void* h) {
-  return guard_ptr([&]() -> void* { return static_cast<Model*>(h)->build().release(); });
+  return guard_ptr([&]() -> void* { return ref<Model>(h)->build().release(); });
 }
 
 // ----------------------------------------------------------------- image
 
 void insem_image_free(void* h) { delete static_cast<SetupImage*>(h); }
 
-const insem_hip_setup* insem_image_view(void* h) { return &static_cast<SetupImage*>(h)->view; }
+const insem_hip_setup* insem_image_view(void* h) {
+  return static_cast<const insem_hip_setup*>(guard_ptr([&]() -> void* { return &ref<SetupImage>(h)->view; }));
+}
 
 int insem_image_domain_info(void* h, int32_t di, int32_t* out) {
   return guard([&] {
-    const SetupImage& im = *static_cast<SetupImage*>(h);
+    const SetupImage& im = *ref<SetupImage>(h);
     INSEM_REQUIRE(di >= 0 && di < static_cast<int>(im.domains.size()), "invalid domain " << di);
     const auto& d = im.domains[di];
     out[0] = static_cast<int32_t>(d.medium);
@@ -285,7 +296,7 @@ int insem_image_domain_info(void* h, int32_t di, int32_t* out) {
 
 int insem_image_block_info(void* h, int32_t di, int32_t bi, int32_t* out) {
   return guard([&] {
-    const auto& d = static_cast<SetupImage*>(h)->domains.at(di);
+    const auto& d = ref<SetupImage>(h)->domains.at(di);
     const auto& b = d.blocks.at(bi);
     out[0] = b.kind;
     out[1] = static_cast<int32_t>(b.elements.size());
@@ -293,47 +304,54 @@ int insem_image_block_info(void* h, int32_t di, int32_t bi, int32_t* out) {
   });
 }
 
-int insem_image_n_domains(void* h) { return static_cast<int>(static_cast<SetupImage*>(h)->domains.size()); }
+int insem_image_n_domains(void* h) {
+  int n = -1;
+  (void)guard([&] { n = static_cast<int>(ref<SetupImage>(h)->domains.size()); });
+  return n;
+}
 
 int insem_image_coords(void* h, int32_t di, double* out) {
   return guard([&] {
-    const auto& d = static_cast<SetupImage*>(h)->domains.at(di);
+    const auto& d = ref<SetupImage>(h)->domains.at(di);
     std::memcpy(out, d.coords.data(), d.coords.size() * sizeof(double));
   });
 }
 
 int insem_image_mass(void* h, int32_t di, double* out) {
   return guard([&] {
-    const auto& d = static_cast<SetupImage*>(h)->domains.at(di);
+    const auto& d = ref<SetupImage>(h)->domains.at(di);
     for (std::size_t g = 0; g < d.mass.size(); ++g) out[g] = d.mass[g];
   });
 }
 
 int64_t insem_image_start_len(void* h) {
-  const SetupImage& im = *static_cast<SetupImage*>(h);
-  int64_t n = 0;
-  for (const auto& d : im.domains) n += static_cast<int64_t>(d.ncomp) * d.dofs.n_global;
-  if (!im.coupling.fluid_points.empty()) n *= 2;
+  int64_t n = -1;
+  (void)guard([&] {
+    const SetupImage& im = *ref<SetupImage>(h);
+    n = 0;
+    for (const auto& d : im.domains) n += static_cast<int64_t>(d.ncomp) * d.dofs.n_global;
+    if (!im.coupling.fluid_points.empty()) n *= 2;
+  });
   return n;
 }
 
 int insem_image_start_vectors(void* h, float* out) {
   return guard([&] {
-    const auto v = static_cast<SetupImage*>(h)->power_start_vectors();
+    const auto v = ref<SetupImage>(h)->power_start_vectors();
     std::memcpy(out, v.data(), v.size() * sizeof(float));
   });
 }
 
 int insem_image_amplitudes(void* h, double dt, int32_t n_steps, float* out) {
   return guard([&] {
-    const auto a = static_cast<SetupImage*>(h)->source_amplitudes(dt, n_steps);
+    const auto a = ref<SetupImage>(h)->source_amplitudes(dt, n_steps);
     std::memcpy(out, a.data(), a.size() * sizeof(float));
   });
 }
 
 int insem_image_time(void* h, double* d, int32_t* i) {
   return guard([&] {
-    const TimeConfig& t = static_cast<SetupImage*>(h)->time;
+    const TimeConfig& t = ref<SetupImage>(h)->time;
     d[0] = t.dt;
     d[1] = t.safety;
     d[2] = t.duration;
@@ -347,7 +365,7 @@ int insem_image_time(void* h, double* d, int32_t* i) {
 // would): d = dt, stable_dt; *n = n_steps.
 int insem_image_model_time(void* h, double lambda, double* d, int32_t* n) {
   return guard([&] {
-    const ModelTime m = model_time(static_cast<SetupImage*>(h)->time, lambda);
+    const ModelTime m = model_time(ref<SetupImage>(h)->time, lambda);
     d[0] = m.dt;
     d[1] = m.stable_dt;
     *n = m.n_steps;
@@ -355,8 +373,11 @@ int insem_image_model_time(void* h, double lambda, double* d, int32_t* n) {
 }
 
 int insem_image_n_traces(void* h) {
-  int n = 0;
-  for (const auto& r : static_cast<SetupImage*>(h)->receivers) n += static_cast<int>(r.quantities.size());
+  int n = -1;
+  (void)guard([&] {
+    n = 0;
+    for (const auto& r : ref<SetupImage>(h)->receivers) n += static_cast<int>(r.quantities.size());
+  });
   return n;
 }
 
@@ -364,7 +385,7 @@ int insem_image_trace_info(void* h, int32_t t, char* name, int32_t name_len, cha
                            int32_t* ncomp) {
   return guard([&] {
     int k = 0;
-    for (const auto& r : static_cast<SetupImage*>(h)->receivers)
+    for (const auto& r : ref<SetupImage>(h)->receivers)
       for (size_t q = 0; q < r.quantities.size(); ++q, ++k)
         if (k == t) {
           std::snprintf(name, name_len, "%s", r.name.c_str());
@@ -378,7 +399,7 @@ int insem_image_trace_info(void* h, int32_t t, char* name, int32_t name_len, cha
 
 int insem_image_counts(void* h, int64_t* out) {
   return guard([&] {
-    const SetupImage& im = *static_cast<SetupImage*>(h);
+    const SetupImage& im = *ref<SetupImage>(h);
     out[0] = im.total_points();
     out[1] = im.total_dofs();
     out[2] = static_cast<int64_t>(im.coupling.fluid_points.size());
@@ -398,13 +419,17 @@ void* insem_solver_new(void* image) {
 
 void insem_solver_free(void* h) { delete static_cast<Solver*>(h); }
 
-void* insem_solver_image(void* h) { return const_cast<SetupImage*>(&static_cast<Solver*>(h)->image()); }
+void* insem_solver_image(void* h) {
+  return guard_ptr([&]() -> void* { return const_cast<SetupImage*>(&ref<Solver>(h)->image()); });
+}
 
-void* insem_solver_context(void* h) { return static_cast<Solver*>(h)->context(); }
+void* insem_solver_context(void* h) {
+  return guard_ptr([&]() -> void* { return ref<Solver>(h)->context(); });
+}
 
 int insem_solver_info(void* h, double* d, int32_t* i) {
   return guard([&] {
-    const Solver& s = *static_cast<Solver*>(h);
+    const Solver& s = *ref<Solver>(h);
     d[0] = s.dt();
     d[1] = s.stable_dt();
     i[0] = s.n_steps();
@@ -413,33 +438,33 @@ int insem_solver_info(void* h, double* d, int32_t* i) {
 }
 
 int insem_solver_advance(void* h, int32_t n) {
-  return guard([&] { static_cast<Solver*>(h)->advance(n); });
+  return guard([&] { ref<Solver>(h)->advance(n); });
 }
 
 int insem_solver_set_state(void* h, int32_t medium, const double* u, const double* v, int64_t n) {
   return guard([&] {
-    static_cast<Solver*>(h)->set_state(medium == 0 ? Medium::Acoustic : Medium::Elastic, std::vector<double>(u, u + n),
+    ref<Solver>(h)->set_state(medium == 0 ? Medium::Acoustic : Medium::Elastic, std::vector<double>(u, u + n),
                                        std::vector<double>(v, v + n));
   });
 }
 
 int insem_solver_field(void* h, int32_t di, int32_t which, double* out) {
   return guard([&] {
-    const auto f = static_cast<Solver*>(h)->field(di, which);
+    const auto f = ref<Solver>(h)->field(di, which);
     std::memcpy(out, f.data(), f.size() * sizeof(double));
   });
 }
 
 int insem_solver_n_samples(void* h) {
-  int32_t n = 0;
-  insem_hip_n_samples(static_cast<Solver*>(h)->context(), &n);
+  int32_t n = -1;
+  (void)guard([&] { insem_hip_n_samples(ref<Solver>(h)->context(), &n); });
   return n;
 }
 
 int insem_solver_traces(void* h, double* out) {
   return guard([&] {
     std::size_t o = 0;
-    for (const auto& t : static_cast<Solver*>(h)->traces()) {
+    for (const auto& t : ref<Solver>(h)->traces()) {
       std::memcpy(out + o, t.data.data(), t.data.size() * sizeof(double));
       o += t.data.size();
     }
@@ -447,14 +472,14 @@ int insem_solver_traces(void* h, double* out) {
 }
 
 int insem_solver_n_energy(void* h) {
-  int32_t n = 0;
-  insem_hip_n_energy(static_cast<Solver*>(h)->context(), &n);
+  int32_t n = -1;
+  (void)guard([&] { insem_hip_n_energy(ref<Solver>(h)->context(), &n); });
   return n;
 }
 
 int insem_solver_energy(void* h, double* out) {
   return guard([&] {
-    const auto e = static_cast<Solver*>(h)->energy();
+    const auto e = ref<Solver>(h)->energy();
     for (std::size_t i = 0; i < e.size(); ++i)
       for (int k = 0; k < 3; ++k) out[3 * i + k] = e[i][k];
   });
@@ -520,16 +545,18 @@ void* insem_power_solver(void* job, void* image) {
   });
 }
 
-void* insem_batch_context(void* h) { return static_cast<BatchSolver*>(h)->context(); }
+void* insem_batch_context(void* h) {
+  return guard_ptr([&]() -> void* { return ref<BatchSolver>(h)->context(); });
+}
 
 void* insem_batch_model_image(void* h, int32_t m) {
-  return const_cast<SetupImage*>(&static_cast<BatchSolver*>(h)->model(m));
+  return guard_ptr([&]() -> void* { return const_cast<SetupImage*>(&ref<BatchSolver>(h)->model(m)); });
 }
 
 // d: dt, stable_dt; i: n_steps, samples so far.  m = -1: n_models, max steps, union step.
 int insem_batch_info(void* h, int32_t m, double* d, int32_t* i) {
   return guard([&] {
-    const BatchSolver& b = *static_cast<BatchSolver*>(h);
+    const BatchSolver& b = *ref<BatchSolver>(h);
     if (m < 0) {
       i[0] = b.n_models();
       i[1] = b.max_steps();
@@ -545,12 +572,12 @@ int insem_batch_info(void* h, int32_t m, double* d, int32_t* i) {
 }
 
 int insem_batch_advance(void* h, int32_t n) {
-  return guard([&] { static_cast<BatchSolver*>(h)->advance(n); });
+  return guard([&] { ref<BatchSolver>(h)->advance(n); });
 }
 
 int insem_batch_traces(void* h, int32_t m, double* out) {
   return guard([&] {
-    const BatchSolver& b = *static_cast<BatchSolver*>(h);
+    const BatchSolver& b = *ref<BatchSolver>(h);
     INSEM_REQUIRE(m >= 0 && m < b.n_models(), "no model " << m << " in the batch");
     std::size_t o = 0;
     for (const auto& t : b.traces(m)) {
@@ -562,7 +589,7 @@ int insem_batch_traces(void* h, int32_t m, double* out) {
 
 int insem_batch_traces_done(void* h, int32_t m, double* out) {
   return guard([&] {
-    const BatchSolver& b = *static_cast<BatchSolver*>(h);
+    const BatchSolver& b = *ref<BatchSolver>(h);
     INSEM_REQUIRE(m >= 0 && m < b.n_models(), "no model " << m << " in the batch");
     std::size_t o = 0;
     for (const auto& t : b.traces_done(m)) {
@@ -573,12 +600,12 @@ int insem_batch_traces_done(void* h, int32_t m, double* out) {
 }
 
 int insem_batch_device_step(void* h, int32_t* step) {
-  return guard([&] { *step = static_cast<BatchSolver*>(h)->device_step(); });
+  return guard([&] { *step = ref<BatchSolver>(h)->device_step(); });
 }
 
 int64_t insem_batch_footprint(void* h) {
   try {
-    return static_cast<BatchSolver*>(h)->footprint();
+    return ref<BatchSolver>(h)->footprint();
   } catch (...) {
     return -1;
   }

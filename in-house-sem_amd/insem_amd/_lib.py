@@ -172,6 +172,13 @@ def check(rc: int) -> None:
         raise InsemError(host().insem_host_last_error().decode())
 
 
+def check_count(n: int) -> int:
+    """A count or flag from the host library (negative on failure)."""
+    if n < 0:
+        raise InsemError(host().insem_host_last_error().decode())
+    return n
+
+
 def check_ptr(p):
     if not p:
         raise InsemError(host().insem_host_last_error().decode())

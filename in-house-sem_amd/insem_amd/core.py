@@ -14,7 +14,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _lib
-from ._lib import InsemError, check, check_ptr
+from ._lib import InsemError, check, check_count, check_ptr
 
 MEDIA = {"acoustic": 0, "elastic": 1}
 BOUNDARY_TYPES = {"free": 0, "rigid": 1, "absorbing": 2}
@@ -80,7 +80,7 @@ class MacroMesh:
         return float(out[0]), float(out[1])
 
     def is_axis_face(self, cell: int, face: int) -> bool:
-        return bool(_lib.host().insem_mesh_is_axis_face(self._h, cell, face))
+        return bool(check_count(_lib.host().insem_mesh_is_axis_face(self._h, cell, face)))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -229,8 +229,8 @@ class SetupImage:
         self._h = handle
         self._owned = owned
         lib = _lib.host()
-        self.view = lib.insem_image_view(handle)
-        self.n_domains = lib.insem_image_n_domains(handle)
+        self.view = check_ptr(lib.insem_image_view(handle))
+        self.n_domains = check_count(lib.insem_image_n_domains(handle))
         self.domains = []
         for di in range(self.n_domains):
             info = np.zeros(10, dtype=np.int32)
@@ -257,7 +257,7 @@ class SetupImage:
         self.time = TimeConfig(dt=float(td[0]), safety=float(td[1]), duration=float(td[2]), n_steps=int(ti[0]),
                                record_every=int(ti[1]), energy_every=int(ti[2]))
         self.traces = []
-        for t in range(lib.insem_image_n_traces(handle)):
+        for t in range(check_count(lib.insem_image_n_traces(handle))):
             name = ctypes.create_string_buffer(256)
             q = ctypes.create_string_buffer(64)
             nc = ctypes.c_int32()
@@ -282,7 +282,7 @@ class SetupImage:
         return out
 
     def start_vectors(self) -> np.ndarray:
-        n = _lib.host().insem_image_start_len(self._h)
+        n = check_count(_lib.host().insem_image_start_len(self._h))
         out = np.zeros(n, dtype=np.float32)
         check(_lib.host().insem_image_start_vectors(self._h, _fptr(out)))
         return out
@@ -401,7 +401,7 @@ class Solver:
         check(_lib.host().insem_solver_set_state(self._h, MEDIA[medium], _dptr(u), _dptr(v), u.size))
 
     def _all_traces(self):
-        ns = _lib.host().insem_solver_n_samples(self._h)
+        ns = check_count(_lib.host().insem_solver_n_samples(self._h))
         total = sum(nc for _, _, nc in self.image.traces) * ns
         buf = np.zeros(max(total, 1))
         check(_lib.host().insem_solver_traces(self._h, _dptr(buf)))
@@ -423,12 +423,12 @@ class Solver:
 
     @property
     def times(self) -> np.ndarray:
-        ns = _lib.host().insem_solver_n_samples(self._h)
+        ns = check_count(_lib.host().insem_solver_n_samples(self._h))
         return np.arange(ns) * self.image.time.record_every * self.dt
 
     @property
     def energy(self) -> np.ndarray:
-        n = _lib.host().insem_solver_n_energy(self._h)
+        n = check_count(_lib.host().insem_solver_n_energy(self._h))
         out = np.zeros((max(n, 1), 3))
         check(_lib.host().insem_solver_energy(self._h, _dptr(out)))
         return out[:n]

@@ -154,3 +154,17 @@ def test_setup_counts_of_layered_coupled_model():
     assert img.n_coupling == 80 * 4 + 1
     assert f["n_dirichlet"] == 80 * 4 + 1  # free top of the fluid
     assert sim.geometry_kind_counts() == {"rect": 80 * 8, "affine": 0, "curved": 80 * 32}
+
+
+def test_closed_or_taken_over_handles_raise_instead_of_crashing():
+    # a batch takes its images over (their handles become None); any later
+    # use of such an image, or of a closed one, is an InsemError
+    from insem_amd.configs import layered_coupled
+
+    img = layered_coupled(nx=8, nz_fluid=2, nz_solid=(2, 2), duration=1e-3).build_setup()
+    assert img.time_step(4.0e8)[1] > 0
+    img._owned, img._h = False, None  # what BatchSolver does to the images it takes
+    for call in (lambda: img.time_step(4.0e8), img.start_vectors, lambda: img.mass(0), lambda: img.coordinates(0),
+                 lambda: img.amplitudes(1e-6, 3), lambda: insem.SetupImage(None)):
+        with pytest.raises(insem.InsemError, match="null handle"):
+            call()
