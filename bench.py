@@ -273,7 +273,7 @@ def batched_leg(n_models, duration, plain_sample, batch_size=5, seed=2026):
             "batches": n_batches}
 
 
-def served_leg(n_requests, rate, sweep, slots=4, batch=8, seed=7):
+def served_leg(n_requests, rate, sweep, slots=3, batch=16, seed=7):
     """A Poisson stream of mixed requests (60 % cased-hole dipole, 30 %
     open-hole monopole at 3-8 ms; 10 % 2D layered coupled at 1-4 M nodes for
     0.5-1 s) served by the continuous-batching loop at each arrival rate of
@@ -364,8 +364,8 @@ def main():
                     help="Poisson arrival rate of the served leg's comparison with one-at-a-time service (1/s)")
     ap.add_argument("--serve-sweep", default="8",
                     help="arrival rates (1/s) the served loop runs at, comma-separated (e.g. 2,4,8)")
-    ap.add_argument("--serve-slots", type=int, default=4, help="batches the served loop keeps on the device at once")
-    ap.add_argument("--serve-batch", type=int, default=8, help="requests per batch of the served loop")
+    ap.add_argument("--serve-slots", type=int, default=3, help="batches the served loop keeps on the device at once")
+    ap.add_argument("--serve-batch", type=int, default=16, help="requests per batch of the served loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every run of this build uses a single MI355X "

@@ -185,7 +185,7 @@ class Server:
     offsets are the receivers' z), or None.  timeout: seconds after its
     arrival by which a request must have finished, or None."""
 
-    def __init__(self, max_slots: int = 4, max_batch: int = 16, chunk: int = 1024, lookahead: int = 2,
+    def __init__(self, max_slots: int = 3, max_batch: int = 16, chunk: int = 1024, lookahead: int = 2,
                  hbm_budget: Optional[int] = None, stc: Optional[dict] = None, max_large_slots: int = 1,
                  large_dofs: int = 2_000_000, max_power_jobs: int = 4, timeout: Optional[float] = None,
                  workers: int = 4, batch_wait: float = 0.0):
