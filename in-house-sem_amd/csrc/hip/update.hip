@@ -119,17 +119,39 @@ __device__ __forceinline__ int tail_block_model(const DomDev& d, int64_t g0) {
   return m0 == m1 ? m0 : -1;
 }
 
+// The values a regular point's update reads besides its force, loaded
+// before the force is assembled so that both sets of loads are in flight
+// together.
+template <int C>
+struct PointState {
+  float im;
+  float u[C], v[C];
+};
+
+template <int C>
+__device__ __forceinline__ PointState<C> load_point(const DomDev& d, int64_t g) {
+  const uint32_t ng = static_cast<uint32_t>(d.ng), gi = static_cast<uint32_t>(g);
+  PointState<C> s;
+  s.im = *at32(d.im, gi);
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    s.u[c] = *at32(d.u, c * ng + gi);
+    s.v[c] = *at32(d.v, c * ng + gi);
+  }
+  return s;
+}
+
 // a = M^-1 f, optional corrector, energy terms and the next predictor at a
-// regular point g with assembled force f.
+// regular point g with assembled force f and the values st loaded before.
 template <int C>
 __device__ __forceinline__ void update_point(const DomDev& d, const PointStep& ps, int corr, bool energy, int64_t g,
-                                             const float* f, double& ep, double& ek) {
+                                             const float* f, const PointState<C>& st, double& ep, double& ek) {
   const uint32_t ng = static_cast<uint32_t>(d.ng), gi = static_cast<uint32_t>(g);
-  const float im = *at32(d.im, gi);
+  const float im = st.im;
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     const uint32_t o = c * ng + gi;
-    float u = *at32(d.u, o), v = *at32(d.v, o);
+    float u = st.u[c], v = st.v[c];
     const float a = f[c] * im;
     if (energy) ep += static_cast<double>(u) * static_cast<double>(f[c]);
     if (corr) v = v + ps.hdt * a;
@@ -169,9 +191,10 @@ __device__ __forceinline__ void regular_body(const DomDev& d, int64_t first, con
   if (g >= d.n_reg) return;
   const PointStep ps = point_step(d, sc, pred, step, g, tail_block_model(d, g0));
   if (!ps.active) return;
+  const PointState<C> st = load_point<C>(d, g);
   float f[C];
   gather_sum<C, W>(d, g, f);
-  update_point<C>(d, ps, corr, energy, g, f, ep, ek);
+  update_point<C>(d, ps, corr, energy, g, f, st, ep, ek);
 }
 
 // Sums of the K contributions per component of a grouped point, from zero
@@ -204,6 +227,7 @@ __device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc
   if (!ps.active) return;
   const uint32_t o = static_cast<uint32_t>(g - d.grp.begin);
   const uint32_t z = static_cast<uint32_t>(d.grp.end - d.grp.begin), rows = static_cast<uint32_t>(d.grp.rows);
+  const PointState<C> st = load_point<C>(d, g);
   float f[C];
   switch (group_count(d.grp, static_cast<uint32_t>(g))) {
     case 2: group_sum<C, 2>(d.contrib, o, z, rows, f); break;
@@ -214,7 +238,7 @@ __device__ __forceinline__ void grouped_body(const DomDev& d, const StepCoef& sc
     case 7: group_sum<C, 7>(d.contrib, o, z, rows, f); break;
     default: group_sum<C, 8>(d.contrib, o, z, rows, f); break;
   }
-  update_point<C>(d, ps, corr, energy, g, f, ep, ek);
+  update_point<C>(d, ps, corr, energy, g, f, st, ep, ek);
 }
 
 // Small SPD solve without pivoting, in double as the reference's
