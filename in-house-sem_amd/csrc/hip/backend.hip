@@ -95,6 +95,10 @@ BlockCache g_cache[kCacheDevices];
 // The device's cache (devices beyond kCacheDevices allocate and free directly).
 BlockCache* cache_of(int device) { return device >= 0 && device < kCacheDevices ? &g_cache[device] : nullptr; }
 
+// Blocks reserved for staged contexts (insem_hip_reserve): address ->
+// (device, size).  Guarded by g_capture_mu.
+std::map<void*, std::pair<int, size_t>> g_reserved;
+
 // Allocation granularity: 4 KiB up to 1 MiB, 2 MiB above.
 size_t block_bytes(size_t bytes) {
   const size_t g = bytes <= (size_t{1} << 20) ? size_t{4} << 10 : size_t{2} << 20;
@@ -214,6 +218,16 @@ struct insem_hip_ctx {
   hipStream_t stream = nullptr;
   std::vector<std::pair<void*, size_t>> allocs;  // device blocks and their sizes
   int64_t footprint = 0;
+  // Staged set-up (insem_hip_stage_size / insem_hip_stage): no device call
+  // is made; the device arrays are laid end to end (256-byte aligned) in one
+  // reserved block at stage_base (a stand-in address while only measuring),
+  // and every upload lands in the host image stage_host at the same offset
+  // for insem_hip_realize to copy.  0: a device context; 1: measuring; 2:
+  // staging into a reserved block.
+  int staging = 0;
+  uintptr_t stage_base = 0;
+  size_t stage_cap = 0, stage_used = 0;
+  std::vector<unsigned char> stage_host;
   int ngll = 0;
   float *D = nullptr, *Dj = nullptr, *w2 = nullptr;
   std::vector<DomainRT> doms;
@@ -278,12 +292,24 @@ struct insem_hip_ctx {
   double force_ms = 0.0, update_ms = 0.0;
   int force_n = 0, update_n = 0;
 
+  void* stage_alloc(size_t bytes) {
+    const size_t off = (stage_used + 255) / 256 * 256;
+    stage_used = off + bytes;
+    footprint += static_cast<int64_t>(bytes);
+    if (staging == 2 && stage_used > stage_cap) {
+      set_error(INSEM_HIP_EINVAL, "the staged set-up needs more than the %zu bytes reserved for it", stage_cap);
+      throw HipFailure{INSEM_HIP_EINVAL};
+    }
+    constexpr uintptr_t kMeasureBase = uintptr_t{1} << 20;  // never dereferenced
+    return reinterpret_cast<void*>((staging == 2 ? stage_base : kMeasureBase) + off);
+  }
   // n values on the context's device; a block that held another
   // context's data is cleared first unless `clear` is false (the caller
   // writes all n values before any kernel reads them).
   template <class T>
   T* alloc(size_t n, bool clear = true) {
     if (n == 0) n = 1;
+    if (staging) return static_cast<T*>(stage_alloc(n * sizeof(T)));  // the host image starts zeroed
     size_t size = 0;
     bool reused = false;
     void* p = take_block(device, n * sizeof(T), &size, &reused);
@@ -298,6 +324,10 @@ struct insem_hip_ctx {
   // set up on several threads at once).
   void h2d(void* d, const void* src, size_t bytes) {
     if (!bytes) return;
+    if (staging) {
+      if (staging == 2) std::memcpy(stage_host.data() + (reinterpret_cast<uintptr_t>(d) - stage_base), src, bytes);
+      return;
+    }
     HIP_TRY(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, stream));
     HIP_TRY(hipStreamSynchronize(stream));
   }
@@ -306,7 +336,13 @@ struct insem_hip_ctx {
     HIP_TRY(hipMemcpyAsync(d, src, bytes, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
   }
-  void zero(void* d, size_t bytes) { HIP_TRY(hipMemsetAsync(d, 0, bytes, stream)); }
+  void zero(void* d, size_t bytes) {
+    if (staging) {
+      if (staging == 2) std::memset(stage_host.data() + (reinterpret_cast<uintptr_t>(d) - stage_base), 0, bytes);
+      return;
+    }
+    HIP_TRY(hipMemsetAsync(d, 0, bytes, stream));
+  }
   template <class T>
   T* upload(const T* src, size_t n) {
     T* d = alloc<T>(n, n == 0);
@@ -1230,6 +1266,7 @@ void require_time(insem_hip_ctx* c) {
 template <class F>
 int guarded(insem_hip_ctx* c, F&& f) {
   if (!c) return set_error(INSEM_HIP_EINVAL, "null context");
+  if (c->staging) return set_error(INSEM_HIP_ESTATE, "the context is staged: insem_hip_realize it first");
   try {
     if (c->stream) HIP_TRY(hipSetDevice(c->device));
     f();
@@ -1339,7 +1376,8 @@ int insem_hip_set_device(int32_t device) {
 }
 
 namespace {
-int create_impl(const insem_hip_setup* s, const insem_hip_batch* batch, insem_hip_ctx** out);
+int create_impl(const insem_hip_setup* s, const insem_hip_batch* batch, insem_hip_ctx** out, int staging = 0,
+                void* block = nullptr);
 }
 
 int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) { return create_impl(s, nullptr, out); }
@@ -1347,6 +1385,77 @@ int insem_hip_create(const insem_hip_setup* s, insem_hip_ctx** out) { return cre
 int insem_hip_create_batch(const insem_hip_setup* s, const insem_hip_batch* batch, insem_hip_ctx** out) {
   if (!batch) return set_error(INSEM_HIP_EINVAL, "null batch");
   return create_impl(s, batch, out);
+}
+
+int insem_hip_stage_size(const insem_hip_setup* s, int64_t* bytes) {
+  if (!bytes) return set_error(INSEM_HIP_EINVAL, "null argument");
+  insem_hip_ctx* c = nullptr;
+  const int rc = create_impl(s, nullptr, &c, 1);
+  if (rc != INSEM_HIP_OK) return rc;
+  *bytes = static_cast<int64_t>(c->stage_used);
+  delete c;  // a measured context holds nothing on the device
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_reserve(int64_t bytes, void** block) {
+  if (!block || bytes <= 0) return set_error(INSEM_HIP_EINVAL, "a reservation needs a positive size and an output");
+  *block = nullptr;
+  try {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    size_t size = 0;
+    bool reused = false;
+    void* p = take_block(dev, static_cast<size_t>(bytes), &size, &reused);
+    std::lock_guard<std::mutex> lock(g_capture_mu);
+    g_reserved[p] = {dev, size};
+    *block = p;
+    return INSEM_HIP_OK;
+  } catch (const HipFailure& e) {
+    return e.code;
+  }
+}
+
+int insem_hip_unreserve(void* block) {
+  std::pair<int, size_t> r;
+  {
+    std::lock_guard<std::mutex> lock(g_capture_mu);
+    const auto it = g_reserved.find(block);
+    if (it == g_reserved.end()) return set_error(INSEM_HIP_EINVAL, "the block is not reserved");
+    r = it->second;
+    g_reserved.erase(it);
+  }
+  give_block(r.first, block, r.second);
+  return INSEM_HIP_OK;
+}
+
+int insem_hip_stage(const insem_hip_setup* s, void* block, insem_hip_ctx** out) {
+  return create_impl(s, nullptr, out, 2, block);
+}
+
+int insem_hip_realize(insem_hip_ctx* c) {
+  if (!c) return set_error(INSEM_HIP_EINVAL, "null context");
+  if (c->staging != 2) return set_error(INSEM_HIP_ESTATE, "the context is not a staged one");
+  void* block = reinterpret_cast<void*>(c->stage_base);
+  try {
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipMemcpyAsync(block, c->stage_host.data(), c->stage_used, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } catch (const HipFailure& e) {
+    if (c->stream) {
+      (void)hipStreamDestroy(c->stream);
+      c->stream = nullptr;
+    }
+    return e.code;  // still staged: insem_hip_destroy gives the block back
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_capture_mu);
+    g_reserved.erase(block);
+  }
+  c->allocs.emplace_back(block, c->stage_cap);
+  c->staging = 0;
+  std::vector<unsigned char>().swap(c->stage_host);
+  return INSEM_HIP_OK;
 }
 
 }  // extern "C"
@@ -1445,22 +1554,41 @@ struct BatchOrderedSetup {
   }
 };
 
-int create_impl(const insem_hip_setup* s0, const insem_hip_batch* batch, insem_hip_ctx** out) {
+// A context of the setup (and batch): on the current device (staging 0),
+// or without any device call, measured (1) or laid out for the reserved
+// block `block` (2) for insem_hip_realize.
+int create_impl(const insem_hip_setup* s0, const insem_hip_batch* batch, insem_hip_ctx** out, int staging,
+                void* block) {
   if (!out) return set_error(INSEM_HIP_EINVAL, "null output pointer");
   *out = nullptr;
   int v = validate_setup(s0);
   if (v == INSEM_HIP_OK && batch) v = validate_batch(s0, batch);
   if (v != INSEM_HIP_OK) return v;
-  if (insem_hip_device_count() <= 0) return set_error(INSEM_HIP_EDEVICE, "no HIP device is visible");
+  if (!staging && insem_hip_device_count() <= 0) return set_error(INSEM_HIP_EDEVICE, "no HIP device is visible");
   auto* c = new insem_hip_ctx();
+  c->staging = staging;
+  if (staging == 2) {
+    std::lock_guard<std::mutex> lock(g_capture_mu);
+    const auto it = g_reserved.find(block);
+    if (it == g_reserved.end()) {
+      delete c;
+      return set_error(INSEM_HIP_EINVAL, "the block was not reserved with insem_hip_reserve (or is in use)");
+    }
+    c->device = it->second.first;
+    c->stage_cap = it->second.second;
+    c->stage_base = reinterpret_cast<uintptr_t>(block);
+  }
   try {
+    if (staging == 2) c->stage_host.assign(c->stage_cap, 0);
     // Blocks are launched in batch order unless INSEM_HIP_BATCH_ORDER=0.
     const char* bo = std::getenv("INSEM_HIP_BATCH_ORDER");
     std::unique_ptr<BatchOrderedSetup> ordered;
     if (!bo || std::atoi(bo) != 0) ordered = std::make_unique<BatchOrderedSetup>(s0);
     const insem_hip_setup* s = ordered ? &ordered->setup : s0;
-    HIP_TRY(hipGetDevice(&c->device));
-    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (!staging) {
+      HIP_TRY(hipGetDevice(&c->device));
+      HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    }
     c->ngll = s->ngll;
     const size_t npe = static_cast<size_t>(s->ngll) * s->ngll;
     c->D = c->upload(s->D, npe);
@@ -1500,7 +1628,7 @@ int create_impl(const insem_hip_setup* s0, const insem_hip_batch* batch, insem_h
     }
     if (const char* f = std::getenv("INSEM_HIP_FUSE")) c->fuse = std::atoi(f) != 0;
     if (const char* f = std::getenv("INSEM_HIP_PAIR")) c->pair = std::atoi(f) != 0;
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!staging) HIP_TRY(hipStreamSynchronize(c->stream));
   } catch (const HipFailure& e) {
     free_all(c);
     delete c;
@@ -1559,6 +1687,12 @@ extern "C" {
 
 int insem_hip_destroy(insem_hip_ctx* c) {
   if (!c) return INSEM_HIP_OK;
+  if (c->staging) {
+    // never realized: no device resources but its reserved block
+    if (c->staging == 2) (void)insem_hip_unreserve(reinterpret_cast<void*>(c->stage_base));
+    delete c;
+    return INSEM_HIP_OK;
+  }
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_all(c);

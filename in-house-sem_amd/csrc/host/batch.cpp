@@ -180,6 +180,29 @@ PowerJob::PowerJob(const SetupImage& model) {
   }
 }
 
+StagedPower::StagedPower(const SetupImage& model, void* block) {
+  check_hip(insem_hip_stage(&model.view, block, &ctx));
+  start = model.power_start_vectors();
+}
+
+StagedPower::~StagedPower() {
+  if (ctx) insem_hip_destroy(ctx);
+}
+
+PowerJob::PowerJob(StagedPower& staged) {
+  INSEM_REQUIRE(staged.ctx, "the staged power iteration was already started");
+  check_hip(insem_hip_realize(staged.ctx));
+  ctx_ = staged.ctx;
+  staged.ctx = nullptr;
+  try {
+    check_hip(insem_hip_lambda_start(ctx_, staged.start.data()));
+  } catch (...) {
+    insem_hip_destroy(ctx_);
+    ctx_ = nullptr;
+    throw;
+  }
+}
+
 PowerJob::~PowerJob() {
   if (ctx_) insem_hip_destroy(ctx_);
 }

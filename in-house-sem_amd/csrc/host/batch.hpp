@@ -36,9 +36,24 @@ BatchImage union_of(const std::vector<const SetupImage*>& models);
 // ready() asks whether they have finished, lambda() waits for the result
 // and frees the context.  The model is copied to the device by the
 // constructor and not read afterwards.
+// The host half of a power iteration's start (insem_hip_stage: the
+// context laid out for a reserved block, and the start vectors), made on any
+// thread without device calls; PowerJob(StagedPower&) finishes it.
+struct StagedPower {
+  StagedPower(const SetupImage& model, void* block);
+  ~StagedPower();  // a context never realized gives its block back
+  StagedPower(const StagedPower&) = delete;
+  StagedPower& operator=(const StagedPower&) = delete;
+  insem_hip_ctx* ctx = nullptr;
+  std::vector<float> start;
+};
+
 class PowerJob {
  public:
   explicit PowerJob(const SetupImage& model);
+  // Realizes a staged context (device calls) and starts its iteration; the
+  // staged object keeps nothing afterwards.
+  explicit PowerJob(StagedPower& staged);
   ~PowerJob();
   PowerJob(const PowerJob&) = delete;
   PowerJob& operator=(const PowerJob&) = delete;

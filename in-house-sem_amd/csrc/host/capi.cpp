@@ -517,6 +517,26 @@ void* insem_power_start(void* image) {
   return guard_ptr([&]() -> void* { return new PowerJob(*static_cast<const SetupImage*>(image)); });
 }
 
+// Device bytes of the model's context (host only, any thread).
+int insem_image_stage_size(void* image, int64_t* bytes) {
+  return guard([&] { check_hip(insem_hip_stage_size(&ref<SetupImage>(image)->view, bytes)); });
+}
+
+// The host half of a power iteration's start for a block from
+// insem_hip_reserve (host only, any thread); null on failure (the block
+// then stays reserved).
+void* insem_power_stage(void* image, void* block) {
+  return guard_ptr([&]() -> void* { return new StagedPower(*ref<SetupImage>(image), block); });
+}
+
+// Device half: realizes the staged context and starts the iteration.
+void* insem_power_start_staged(void* staged) {
+  return guard_ptr([&]() -> void* { return new PowerJob(*ref<StagedPower>(staged)); });
+}
+
+// Frees a staged start (a context never realized gives its block back).
+void insem_power_staged_free(void* staged) { delete static_cast<StagedPower*>(staged); }
+
 // *ready = 1 when the iteration has finished (lambda then returns at once).
 int insem_power_ready(void* job, int32_t* ready) {
   return guard([&] { *ready = static_cast<PowerJob*>(job)->ready() ? 1 : 0; });

@@ -50,6 +50,11 @@ def hip():
         _declare(lib, "insem_hip_set_device", c_int, c_int)
         _declare(lib, "insem_hip_create", c_int, c_void, P(c_void))
         _declare(lib, "insem_hip_destroy", c_int, c_void)
+        _declare(lib, "insem_hip_stage_size", c_int, c_void, P(c_i64))
+        _declare(lib, "insem_hip_reserve", c_int, c_i64, P(c_void))
+        _declare(lib, "insem_hip_unreserve", c_int, c_void)
+        _declare(lib, "insem_hip_stage", c_int, c_void, c_void, P(c_void))
+        _declare(lib, "insem_hip_realize", c_int, c_void)
         _declare(lib, "insem_hip_set_time", c_int, c_void, c_double, c_int, P(ctypes.c_float))
         _declare(lib, "insem_hip_lambda_max", c_int, c_void, P(ctypes.c_float), P(c_double))
         _declare(lib, "insem_hip_lambda_start", c_int, c_void, P(ctypes.c_float))
@@ -151,6 +156,10 @@ def host():
         d("insem_batch_new_lambdas", c_void, P(c_void), P(c_double), c_int)
         d("insem_batch_free", None, c_void)
         d("insem_power_start", c_void, c_void)
+        d("insem_image_stage_size", c_int, c_void, P(c_i64))
+        d("insem_power_stage", c_void, c_void, c_void)
+        d("insem_power_start_staged", c_void, c_void)
+        d("insem_power_staged_free", None, c_void)
         d("insem_power_ready", c_int, c_void, P(c_int))
         d("insem_power_lambda", c_int, c_void, P(c_double))
         d("insem_power_free", None, c_void)

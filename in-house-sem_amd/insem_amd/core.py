@@ -287,6 +287,13 @@ class SetupImage:
         check(_lib.host().insem_image_start_vectors(self._h, _fptr(out)))
         return out
 
+    def stage_size(self) -> int:
+        """Device bytes of this model's context (host work only, any thread:
+        insem_hip_stage_size)."""
+        n = ctypes.c_int64()
+        check(_lib.host().insem_image_stage_size(self._h, ctypes.byref(n)))
+        return int(n.value)
+
     def time_step(self, lambda_max: float) -> Tuple[float, int]:
         """(dt, n_steps) the model gets from lambda_max, as its Solver would
         (reference: solver.cpp:131-144)."""
@@ -446,6 +453,41 @@ class Solver:
 # ------------------------------------------------------------------ batch
 
 
+def reserve_block(n_bytes: int) -> int:
+    """A device block of at least n_bytes on the current device for a
+    staged context (insem_hip_reserve); its address."""
+    p = ctypes.c_void_p()
+    _lib.check_hip(_lib.hip().insem_hip_reserve(int(n_bytes), ctypes.byref(p)))
+    return int(p.value)
+
+
+def unreserve_block(block: int) -> None:
+    """Gives back a reserved block that no context took (insem_hip_unreserve)."""
+    _lib.check_hip(_lib.hip().insem_hip_unreserve(ctypes.c_void_p(block)))
+
+
+class StagedPower:
+    """The host half of a PowerJob's start: the model's context laid out for
+    a block from reserve_block(), and its start vectors.  Makes no device
+    call, so it may run on any thread; PowerJob.from_staged() finishes it on
+    the thread that makes the device calls.  If it fails, the block stays
+    reserved."""
+
+    def __init__(self, image: "SetupImage", block: int):
+        self._h = check_ptr(_lib.host().insem_power_stage(image._h, ctypes.c_void_p(block)))
+
+    def close(self):
+        """Frees it; a context never started gives its block back (a device
+        call: on the device-calling thread)."""
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.host().insem_power_staged_free(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 class PowerJob:
     """A model's power iteration (lambda_max of M^-1 K, the one its Solver
     runs for the stable time step) queued on a short-lived device context of
@@ -455,6 +497,15 @@ class PowerJob:
 
     def __init__(self, image: "SetupImage"):
         self._h = check_ptr(_lib.host().insem_power_start(image._h))
+
+    @classmethod
+    def from_staged(cls, staged: StagedPower) -> "PowerJob":
+        """Starts the iteration of a staged context (copies its arrays to
+        the device): the job PowerJob(image) would give, bitwise."""
+        job = cls.__new__(cls)
+        job._h = check_ptr(_lib.host().insem_power_start_staged(staged._h))
+        staged.close()
+        return job
 
     def ready(self) -> bool:
         flag = ctypes.c_int32()

@@ -8,7 +8,10 @@ Requests (a Simulation each) arrive over time and go through four stages:
 2. time step: its power iteration (the lambda_max that fixes its Δt and
    step count, as the request run alone would get them) is queued on the
    device on a short-lived context of its own, and the loop polls it
-   without waiting, so that the running batches keep getting their steps;
+   without waiting, so that the running batches keep getting their steps.
+   The context's set-up is staged: a worker lays it out in host memory for
+   a device block the loop reserved, and the loop only copies it over and
+   queues the iteration;
 3. admission: requests whose time step is known wait for a slot.  Requests
    of at least `large_dofs` degrees of freedom (the 2D layered models) have
    a slot class of their own (`max_large_slots` slots, one request each),
@@ -38,9 +41,9 @@ not finite (a model that diverged) and a request still unfinished `timeout`
 seconds after its arrival (its batch is stopped when none of its requests
 is left) all set the request's `error` and its `done` time.
 
-Host work that does not touch the device (setups, the coherence of finished
-traces) runs on worker threads; every device call stays on the loop's
-thread.
+Host work that does not touch the device (setups, staged contexts, the
+coherence of finished traces) runs on worker threads; every device call
+stays on the loop's thread.
 """
 
 from __future__ import annotations
@@ -56,7 +59,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _lib, post
-from .core import BatchSolver, PowerJob, SetupImage, Simulation
+from .core import BatchSolver, PowerJob, SetupImage, Simulation, StagedPower, reserve_block, unreserve_block
 
 
 @dataclass
@@ -72,6 +75,7 @@ class Request:
     lam: float = float("nan")           # lambda_max of its power iteration
     ready: float = float("nan")         # when its time step was known
     job: object = None                  # a large request's finished power job (its context)
+    stage_bytes: int = 0                # device bytes of its context (for the staged start)
     admitted: float = float("nan")
     done: float = float("nan")
     dt: float = float("nan")
@@ -175,6 +179,26 @@ class _Slot:
 _FAILED = object()
 
 
+class _StagedStart:
+    """A power iteration's start in two halves (Server._start_power)."""
+
+    def __init__(self, server: "Server", pool: ThreadPoolExecutor, r: Request):
+        self.server = server
+        self.block = reserve_block(r.stage_bytes + 65536)  # slack past the last array, as separate blocks have
+        self.future = pool.submit(StagedPower, r.image, self.block)
+
+    def poll(self):
+        if not self.future.done():
+            return None
+        try:
+            staged = self.future.result()
+        except Exception:
+            unreserve_block(self.block)
+            raise
+        with self.server._timed("power_start_s"):
+            return PowerJob.from_staged(staged)
+
+
 class Server:
     """Serves a stream of requests; see the module docstring.
 
@@ -232,21 +256,22 @@ class Server:
     def _solo(self, job, image: SetupImage):
         return _Solo(job.into_solver(image))
 
-    def _start_power(self, image: SetupImage) -> Future:
-        """The power iteration's start (its context's set-up: host work and
-        uploads), on the loop thread: device calls stay on the thread that
-        makes every other device call of the loop (contexts set up on
-        worker threads while the loop replays graphs have stalled)."""
-        f: Future = Future()
-        try:
-            with self._timed("power_start_s"):
-                f.set_result(self._power_job(image))
-        except Exception as e:  # noqa: BLE001 - reported through the future
-            f.set_exception(e)
-        return f
+    def _stage_size(self, image: SetupImage) -> int:
+        return image.stage_size()
+
+    def _start_power(self, pool: ThreadPoolExecutor, r: Request):
+        """A power iteration's start, polled by the loop (poll() returns the
+        job once started): the context's host set-up on a worker, which makes
+        no device call, then its copy to the device and the iteration's
+        launches on the loop thread (device calls stay on the thread that
+        makes every other device call of the loop: contexts set up with
+        device calls on worker threads while the loop replays graphs have
+        stalled)."""
+        return _StagedStart(self, pool, r)
 
     def _prepare(self, r: Request) -> Request:
         r.image = self._setup(r)
+        r.stage_bytes = self._stage_size(r.image)
         r.key = batch_key(r.image)
         r.footprint = footprint_estimate(r.image)
         r.n_dofs = r.image.n_dofs
@@ -404,23 +429,31 @@ class Server:
                 # without waiting
                 while timing and len(power) < self.max_power_jobs and used + timing[0].footprint <= self.hbm_budget:
                     r = timing.popleft()
-                    power.append((r, self._start_power(r.image)))
-                    used += r.footprint
-                    busy = True
-                for r, started in list(power):
-                    if not started.done():
-                        continue
                     try:
-                        job = started.result()
+                        power.append([r, self._start_power(pool, r), None])
+                        used += r.footprint
                     except Exception as e:  # noqa: BLE001 - the request fails, the loop goes on
-                        power.remove((r, started))
-                        used -= r.footprint
                         self._fail(r, f"power iteration failed: {e}", t0)
                         r.image = None
-                        continue
+                    busy = True
+                for entry in list(power):
+                    r, start, job = entry
+                    if job is None:
+                        try:
+                            job = start.poll()
+                        except Exception as e:  # noqa: BLE001 - the request fails, the loop goes on
+                            power.remove(entry)
+                            used -= r.footprint
+                            self._fail(r, f"power iteration failed: {e}", t0)
+                            r.image = None
+                            continue
+                        if job is None:
+                            continue
+                        entry[2] = job
+                        busy = True
                     if not job.ready():
                         continue
-                    power.remove((r, started))
+                    power.remove(entry)
                     if r.large:
                         # the job's context becomes the request's solver: its
                         # bytes stay counted until the slot closes

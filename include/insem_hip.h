@@ -198,6 +198,32 @@ int insem_hip_create(const insem_hip_setup* setup, insem_hip_ctx** out);
 int insem_hip_destroy(insem_hip_ctx* ctx);
 
 /*
+ * insem_hip_create in two halves, so that a host thread that must not stall
+ * (a request loop) makes only the short device calls and the set-up's host
+ * work runs elsewhere.  No counterpart in the reference (its Solver
+ * constructor builds host vectors, solver.cpp:91-145).
+ *   insem_hip_stage_size   host only: the device bytes a context of the
+ *                          setup takes (its arrays laid end to end).
+ *   insem_hip_reserve      device: a block of at least that many bytes on
+ *                          the current device (from the block cache).
+ *   insem_hip_stage        host only, any thread: the context laid out for
+ *                          the block, its arrays in a host image; no device
+ *                          call is made.  The block stays reserved if it
+ *                          fails (insem_hip_unreserve gives it back).
+ *   insem_hip_realize      device: copies the image into the block; the
+ *                          context is then what insem_hip_create returns
+ *                          (bitwise the same arrays).  A staged context
+ *                          refuses every other call until then, and
+ *                          insem_hip_destroy of a staged context gives its
+ *                          block back.
+ */
+int insem_hip_stage_size(const insem_hip_setup* setup, int64_t* bytes);
+int insem_hip_reserve(int64_t bytes, void** block);
+int insem_hip_unreserve(void* block);
+int insem_hip_stage(const insem_hip_setup* setup, void* block, insem_hip_ctx** out);
+int insem_hip_realize(insem_hip_ctx* ctx);
+
+/*
  * Time step, step count and source amplitudes.  amplitudes is
  * [n_sources][n_steps + 1], amplitudes[s][n] = float(stf_s(n * dt)), the value
  * the reference adds at step n (solver.cpp:681).  Must be called before the

@@ -101,6 +101,14 @@ class FakeBatch:
         self.log.append(("closed", [im.spec["seed"] for im in self.models]))
 
 
+class Started:
+    def __init__(self, job):
+        self.job = job
+
+    def poll(self):
+        return self.job
+
+
 class FakeServer(serve.Server):
     def __init__(self, **kw):
         kw.setdefault("hbm_budget", 1 << 40)
@@ -115,6 +123,14 @@ class FakeServer(serve.Server):
     def _power_job(self, img):
         self.log.append(("power", img.spec["seed"]))
         return FakeJob(img, self.log)
+
+    def _stage_size(self, img):
+        return 0
+
+    def _start_power(self, pool, r):
+        # started at once on the loop thread, so that the polls alone order
+        # the requests
+        return Started(self._power_job(r.image))
 
 
     def _batch(self, images, lambdas):
@@ -324,3 +340,49 @@ def test_power_iterations_start_on_the_loop_thread():
     for r in (reqs[0], reqs[2], reqs[3]):
         check_solo(r)
     assert reqs[1].error.startswith("power iteration failed")
+
+
+def test_staged_start_splits_host_and_device_work(monkeypatch):
+    # the server's own start: the block is reserved and the staged context
+    # started on the loop thread, the host half staged on a worker; a
+    # staging that fails gives the block back and fails its request only
+    loop = threading.get_ident()
+    calls = []
+
+    def reserve(n):
+        calls.append(("reserve", threading.get_ident()))
+        return 4096 + len(calls)
+
+    def unreserve(block):
+        calls.append(("unreserve", threading.get_ident(), block))
+
+    class FakeStaged:
+        def __init__(self, image, block):
+            calls.append(("stage", threading.get_ident()))
+            if image.spec.get("stage_fails"):
+                raise RuntimeError("staging failed")
+            self.image = image
+
+    class FakePowerJob:
+        @staticmethod
+        def from_staged(staged):
+            calls.append(("start", threading.get_ident()))
+            return FakeJob(staged.image, [])
+
+    monkeypatch.setattr(serve, "reserve_block", reserve)
+    monkeypatch.setattr(serve, "unreserve_block", unreserve)
+    monkeypatch.setattr(serve, "StagedPower", FakeStaged)
+    monkeypatch.setattr(serve, "PowerJob", FakePowerJob)
+
+    class Real(FakeServer):
+        def _start_power(self, pool, r):
+            return serve.Server._start_power(self, pool, r)
+
+    srv = Real(max_slots=2, max_batch=2, chunk=50)
+    reqs = srv.serve([(0.0, spec(0)), (0.0, spec(1, stage_fails=True)), (0.0, spec(2))])
+    for r in (reqs[0], reqs[2]):
+        check_solo(r)
+    assert reqs[1].error.startswith("power iteration failed")
+    assert all(c[1] == loop for c in calls if c[0] in ("reserve", "unreserve", "start"))
+    assert all(c[1] != loop for c in calls if c[0] == "stage")
+    assert sum(c[0] == "unreserve" for c in calls) == 1

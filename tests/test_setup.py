@@ -168,3 +168,23 @@ def test_closed_or_taken_over_handles_raise_instead_of_crashing():
                  lambda: img.amplitudes(1e-6, 3), lambda: insem.SetupImage(None)):
         with pytest.raises(insem.InsemError, match="null handle"):
             call()
+
+
+def test_stage_size_is_host_only_and_refuses_unreserved_blocks():
+    # the staged set-up's measuring pass makes no device call (this runs
+    # without a GPU) and is deterministic; staging needs a reserved block
+    import ctypes
+
+    from insem_amd import _lib
+    from insem_amd.configs import cased_hole, layered_coupled
+
+    for sim in (layered_coupled(nx=40, nz_fluid=4, nz_solid=(8, 8), duration=2e-3),
+                cased_hole(np.random.default_rng(3))):
+        img = sim.build_setup()
+        n = img.stage_size()
+        assert n == img.stage_size()
+        assert n > 4 * img.n_dofs * 4  # at least u, v, a and f of every DOF
+        ctx = ctypes.c_void_p()
+        rc = _lib.hip().insem_hip_stage(ctypes.c_void_p(img.view), ctypes.c_void_p(1 << 20), ctypes.byref(ctx))
+        assert rc != 0 and not ctx.value
+        assert b"not reserved" in _lib.hip().insem_hip_last_error()
