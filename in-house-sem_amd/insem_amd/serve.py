@@ -273,7 +273,8 @@ class Server:
         r.image = self._setup(r)
         r.stage_bytes = self._stage_size(r.image)
         r.key = batch_key(r.image)
-        r.footprint = footprint_estimate(r.image)
+        # the staged size is the exact device size of its context
+        r.footprint = max(footprint_estimate(r.image), r.stage_bytes)
         r.n_dofs = r.image.n_dofs
         r.large = self.large_dofs is not None and r.n_dofs >= self.large_dofs
         return r
