@@ -31,8 +31,8 @@ RUNNER = os.path.join(os.path.dirname(pkg.__file__), "insem-run")
 
 
 def box_case(lib, snapshot_every=0):
-    """configs[0]'s 2D acoustic box at 16 x 16 quads (absorbing sides,
-    monopole, pressure receiver), in either front end."""
+    """The 2D acoustic box (8 x 8 m, order 4) at 16 x 16 quads (absorbing
+    sides, monopole, pressure receiver), in either front end."""
     m = templates.rectangle((-4.0, 4.0), (-4.0, 4.0), nx=16, nz=16)
     mm = core.MacroMesh(m.cell_type, m.coords, m.nodes, m.cells, m.material, m.boundary) if lib is core else m
     sim = lib.Simulation(mm, lib.Discretization(order=4))
