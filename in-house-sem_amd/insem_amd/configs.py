@@ -56,9 +56,9 @@ def quarter_annulus(nr, nt, r0=1.0, r1=2.0):
     return insem.MacroMesh("quad9", "cartesian", np.array(nodes), np.array(cells), np.zeros(len(cells)))
 
 
-def cfg0(duration=4e-3):
+def acoustic_box(duration=4e-3):
     """2D acoustic box, 32x32 quads, order 4, Stacey on all sides, monopole
-    Ricker 400 Hz at the centre, pressure receiver at (1.5, 0) (README.md:85-95)."""
+    Ricker 400 Hz at the centre, pressure receiver at (1.5, 0) (the reference's README.md:85-95)."""
     m = mesh.rectangle((-4.0, 4.0), (-4.0, 4.0), nx=32, nz=32)
     sim = insem.Simulation(m, insem.Discretization(order=4))
     sim.set_material(0, insem.Material.acoustic(vp=1500.0, rho=1000.0))

@@ -111,7 +111,7 @@ def _small_open():
 
 
 MODELS = {
-    "cfg0": lambda: configs.cfg0(),
+    "acoustic_box": lambda: configs.acoustic_box(),
     "layered_40x20": lambda: configs.layered_coupled(nx=40, nz_fluid=4, nz_solid=(8, 8), duration=0.05,
                                                      n_receivers=7, width=100.0),
     "borehole_m0": lambda: _borehole(0),

@@ -16,7 +16,7 @@ import pytest
 
 import insem_amd as insem
 from insem_amd import _lib
-from models import cfg0, grid, layered_coupled
+from models import acoustic_box, grid, layered_coupled
 from oracle_driver import OracleRun
 
 pytestmark = pytest.mark.gpu
@@ -88,8 +88,8 @@ def test_device_visible():
     assert hip.insem_hip_set_device(dev.value) == 0
 
 
-def test_cfg0_stable_dt_matches_oracle():
-    sim = cfg0()
+def test_acoustic_box_stable_dt_matches_oracle():
+    sim = acoustic_box()
     s = sim.solver
     img = sim.build_setup()
     o = OracleRun(img, dt=1e-6)
@@ -97,8 +97,8 @@ def test_cfg0_stable_dt_matches_oracle():
     assert s.stable_dt == pytest.approx(2.0 / np.sqrt(lam), rel=1e-9)
 
 
-def test_cfg0_internal_forces_bitwise():
-    sim = cfg0()
+def test_acoustic_box_internal_forces_bitwise():
+    sim = acoustic_box()
     s = sim.solver
     img = sim.build_setup()
     o = OracleRun(img, dt=s.dt)
@@ -110,8 +110,8 @@ def test_cfg0_internal_forces_bitwise():
     assert dg == pytest.approx(do, rel=1e-12)
 
 
-def test_cfg0_traces_match_oracle():
-    worst, s, o, _ = compare_runs(cfg0())
+def test_acoustic_box_traces_match_oracle():
+    worst, s, o, _ = compare_runs(acoustic_box())
     assert s.step == s.n_steps
     assert worst <= TRACE_RTOL, worst
 
@@ -220,8 +220,8 @@ def test_mixed_kind_blocks_parity():
     assert img.domains[0]["n_blocks"] == 3
 
 
-def test_cfg0_particle_velocity_and_fields():
-    sim = cfg0(duration=2e-3)
+def test_acoustic_box_particle_velocity_and_fields():
+    sim = acoustic_box(duration=2e-3)
     sim.add_receiver("pv", -0.7, 1.1, ["pressure", "particle_velocity"])
     worst, s, o, img = compare_runs(sim)
     assert worst <= TRACE_RTOL, worst

@@ -15,7 +15,7 @@ import pytest
 
 import analytic
 import insem_amd as insem
-from models import cfg0, grid, layered_coupled
+from models import acoustic_box, grid, layered_coupled
 from oracle_driver import OracleRun
 
 FP32_VS_FP64 = 1e-4
@@ -51,10 +51,10 @@ def worst_trace_error(sim):
     return max(analytic.rel_l2(a[k], b[k]) for k in a)
 
 
-@pytest.mark.parametrize("name", ["cfg0", "layered", "borehole_m0", "borehole_m1"])
+@pytest.mark.parametrize("name", ["acoustic_box", "layered", "borehole_m0", "borehole_m1"])
 def test_float_build_within_stated_error_of_double_build(name):
-    # cfg0 4.6e-7; layered 40x20 1.8e-5; boreholes 2.0e-6 (m = 0), 1.2e-5 (m = 1)
-    sim = {"cfg0": cfg0, "layered": lambda: layered_coupled(nx=40, nz_fluid=4, nz_solid=(8, 8), duration=0.3,
+    # acoustic_box 4.6e-7; layered 40x20 1.8e-5; boreholes 2.0e-6 (m = 0), 1.2e-5 (m = 1)
+    sim = {"acoustic_box": acoustic_box, "layered": lambda: layered_coupled(nx=40, nz_fluid=4, nz_solid=(8, 8), duration=0.3,
                                                               n_receivers=5),
            "borehole_m0": lambda: small_borehole(0), "borehole_m1": lambda: small_borehole(1)}[name]()
     err = worst_trace_error(sim)
