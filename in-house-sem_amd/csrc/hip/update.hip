@@ -278,71 +278,109 @@ struct Accel {
   float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
 };
 
+// Everything a special point's update reads, loaded before any of its
+// arithmetic.
+template <int C>
+struct SpecialIn {
+  uint32_t g = 0;
+  int32_t q0 = 0, q1 = 0, ci = -1, ai = -1;
+  uint8_t fl = 0;
+  float u[C], v[C];
+  float im = 0.0f;
+  double mass = 0.0;
+  PointStep ps;
+  double h_damp = 0.0;
+  float cx = 0.0f, cz = 0.0f;
+  uint32_t o = 0;  // the coupling partner's point in the other domain
+  float cf[C * C];
+  float ot[3] = {0.0f, 0.0f, 0.0f};
+  float f[C];
+};
+
+// The loads of special point i, issued level by level (first everything
+// indexed by the point, then what those indices select; coupling operands
+// with a clamped index), so that each wait covers one level of the chains.
+// `other`: read the coupling operand from the other domain (else the caller
+// supplies it to special_finish).
 template <int C, bool W>
-__device__ __forceinline__ Accel special_point(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
-                                               int pred, int step, bool energy, int64_t i, int bm, double& ep,
-                                               double& ek, Operand ot_in) {
+__device__ __forceinline__ void special_load(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int pred,
+                                             int step, int64_t i, int bm, bool other, SpecialIn<C>& in) {
   const uint32_t g = static_cast<uint32_t>(d.n_reg + i), ng = static_cast<uint32_t>(d.ng);
-  // The point's loads are issued level by level (first everything indexed
-  // by the point, then what those indices select; coupling operands with a
-  // clamped index), so that each wait covers one level of the chains; the
-  // arithmetic below keeps the reference's order.
+  in.g = g;
   const int32_t t0 = *at32(d.t_off, g), t1 = *at32(d.t_off, g + 1);
-  const int32_t q0 = sp.src_off[i], q1 = sp.src_off[i + 1];
-  const int32_t ci = sp.cpl_idx[i], ai = sp.abs_idx[i];
-  const uint8_t fl = sp.flags[i];
-  float u[C], v[C], a[C];
+  in.q0 = sp.src_off[i];
+  in.q1 = sp.src_off[i + 1];
+  in.ci = sp.cpl_idx[i];
+  in.ai = sp.abs_idx[i];
+  in.fl = sp.flags[i];
 #pragma unroll
   for (int c = 0; c < C; ++c) {
-    u[c] = *at32(d.u, c * ng + g);
-    v[c] = *at32(d.v, c * ng + g);
+    in.u[c] = *at32(d.u, c * ng + g);
+    in.v[c] = *at32(d.v, c * ng + g);
   }
-  const float im = *at32(d.im, g);
-  const double mass = *at32(d.mass, g);
-  const PointStep ps = point_step(d, sc, pred, step, g, bm);
-  float cx = 0.0f, cz = 0.0f;
-  uint32_t o = 0;
+  in.im = *at32(d.im, g);
+  in.mass = *at32(d.mass, g);
+  in.ps = point_step(d, sc, pred, step, g, bm);
   if (sp.cpl_cx) {
-    const int32_t cq = ci >= 0 ? ci : 0;
-    cx = sp.cpl_cx[cq];
-    cz = sp.cpl_cz[cq];
-    o = static_cast<uint32_t>(sp.cpl_other[cq]);
+    const int32_t cq = in.ci >= 0 ? in.ci : 0;
+    in.cx = sp.cpl_cx[cq];
+    in.cz = sp.cpl_cz[cq];
+    in.o = static_cast<uint32_t>(sp.cpl_other[cq]);
   }
-  float cf[C * C];
-  if (ai >= 0) {
+  if (in.ai >= 0) {
 #pragma unroll
-    for (int r = 0; r < C * C; ++r) cf[r] = sp.abs_coef[static_cast<int64_t>(ai) * C * C + r];
+    for (int r = 0; r < C * C; ++r) in.cf[r] = sp.abs_coef[static_cast<int64_t>(in.ai) * C * C + r];
   }
-  float ot[3] = {0.0f, 0.0f, 0.0f};
-  if (ci >= 0) {
-    if (ot_in.given) {
-      ot[0] = ot_in.v0;
-      ot[1] = ot_in.v1;
-      ot[2] = ot_in.v2;
-    } else {
-      const uint32_t on = static_cast<uint32_t>(sp.other_ng);
-      ot[0] = *at32(sp.other, o);
-      if (sp.cpl_role != 1) {
-        ot[1] = *at32(sp.other, on + o);
-        if (sp.pm) ot[2] = *at32(sp.other, 2 * on + o);
-      }
+  if (other && in.ci >= 0) {
+    const uint32_t on = static_cast<uint32_t>(sp.other_ng);
+    in.ot[0] = *at32(sp.other, in.o);
+    if (sp.cpl_role != 1) {
+      in.ot[1] = *at32(sp.other, on + in.o);
+      if (sp.pm) in.ot[2] = *at32(sp.other, 2 * on + in.o);
     }
   }
-  if (!ps.active) return Accel{};
-  double h_damp = sc.h_damp;
+  in.h_damp = sc.h_damp;
   if (d.models && sc.h_damp > 0.0) {
     if (bm >= 0)
-      h_damp = d.models[bm].h_damp;  // uniform
+      in.h_damp = d.models[bm].h_damp;  // uniform
     else
-      h_damp = d.models[*at32(d.pmodel, g)].h_damp;
+      in.h_damp = d.models[*at32(d.pmodel, g)].h_damp;
   }
-  float f[C];
-  gather_rows<C, W>(d, t0, t1, f);
+  gather_rows<C, W>(d, t0, t1, in.f);
+}
+
+// The update of a special point from its loads: sources, coupling, Stacey,
+// Dirichlet and axis conditions; the pre-predictor values go to the side
+// buffers.  ot_in (when given) is the coupling operand.
+template <int C, bool W>
+__device__ __forceinline__ Accel special_finish(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
+                                                int step, bool energy, int64_t i, SpecialIn<C>& in, double& ep,
+                                                double& ek, Operand ot_in) {
+  if (!in.ps.active) return Accel{};
+  const uint32_t g = in.g, ng = static_cast<uint32_t>(d.ng);
+  const PointStep& ps = in.ps;
+  float* u = in.u;
+  float* v = in.v;
+  float* f = in.f;
+  float a[C];
+  if (ot_in.given) {
+    in.ot[0] = ot_in.v0;
+    in.ot[1] = ot_in.v1;
+    in.ot[2] = ot_in.v2;
+  }
+  const float* ot = in.ot;
+  const float cx = in.cx, cz = in.cz;
+  const int32_t ci = in.ci, ai = in.ai;
+  const uint8_t fl = in.fl;
+  const float im = in.im;
+  const double mass = in.mass;
+  const float* cf = in.cf;
+  const double h_damp = in.h_damp;
 #pragma unroll
   for (int c = 0; c < C; ++c)
     if (energy) ep += static_cast<double>(u[c]) * static_cast<double>(f[c]);
   // Sources, in source order (solver.cpp:679-687).
-  for (int32_t q = q0; q < q1; ++q) {
+  for (int32_t q = in.q0; q < in.q1; ++q) {
     const float amp = sp.amps[static_cast<int64_t>(sp.src_id[q]) * sp.amp_stride + step];
 #pragma unroll
     for (int c = 0; c < C; ++c) f[c] += sp.src_coef[static_cast<int64_t>(q) * C + c] * amp;
@@ -415,6 +453,16 @@ __device__ __forceinline__ Accel special_point(const DomDev& d, const SpecialDev
   return r;
 }
 
+// A special point's update (loads, then arithmetic).
+template <int C, bool W>
+__device__ __forceinline__ Accel special_point(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
+                                               int pred, int step, bool energy, int64_t i, int bm, double& ep,
+                                               double& ek, Operand ot_in) {
+  SpecialIn<C> in;
+  special_load<C, W>(d, sp, sc, pred, step, i, bm, !ot_in.given, in);
+  return special_finish<C, W>(d, sp, sc, corr, step, energy, i, in, ep, ek, ot_in);
+}
+
 template <int C, bool W>
 __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& sp, const StepCoef& sc, int corr,
                                              int pred, int step, bool energy, int vb, double& ep, double& ek) {
@@ -423,6 +471,7 @@ __device__ __forceinline__ void special_body(const DomDev& d, const SpecialDev& 
   const int bm = tail_block_model(d, d.n_reg + static_cast<int64_t>(vb) * kTB);
   special_point<C, W>(d, sp, sc, corr, pred, step, energy, i, bm, ep, ek, Operand{});
 }
+
 
 // Coupled model, solid special point i: the solid point, then, when it is on
 // the interface, its fluid partner with the solid's new acceleration as the
@@ -438,17 +487,31 @@ __device__ __forceinline__ void special_pair_body(const DomDev& ds, const Specia
   if (i >= sps.n_sp) return;
   // (a fluid partner belongs to its solid point's model)
   const int bm = tail_block_model(ds, ds.n_reg + static_cast<int64_t>(vb) * kTB);
-  const Accel as = special_point<CS, WS>(ds, sps, sc, corr, pred, step, energy, i, bm, ep, ek, Operand{});
-  if (!as.active) return;
-  const int32_t ci = sps.cpl_idx[i];
-  if (ci < 0) return;
-  const int64_t i_f = static_cast<int64_t>(sps.cpl_other[ci]) - df.n_reg;
+  SpecialIn<CS> a;
+  special_load<CS, WS>(ds, sps, sc, pred, step, i, bm, true, a);
+  const bool partner = a.ci >= 0;
+  const int64_t i_f = static_cast<int64_t>(a.o) - df.n_reg;
+  SpecialIn<CF> b;
+  // With three solid components the partner's loads go out before the solid
+  // point's arithmetic (only its coupling term waits for the solid's new
+  // acceleration); the tail then needs more registers and runs at five
+  // waves per SIMD, which measured faster than six with the two points one
+  // after the other.  With two components, or a weighted fluid gather, both
+  // points' values at once would spill registers, so the partner loads
+  // after the solid point's update.
+  constexpr bool overlap = CS == 3 && !WF;
+  if constexpr (overlap) {
+    if (partner) special_load<CF, WF>(df, spf, sc, pred, step, i_f, bm, false, b);
+  }
+  const Accel as = special_finish<CS, WS>(ds, sps, sc, corr, step, energy, i, a, ep, ek, Operand{});
+  if (!as.active || !partner) return;
+  if constexpr (!overlap) special_load<CF, WF>(df, spf, sc, pred, step, i_f, bm, false, b);
   Operand op;
   op.given = true;
   op.v0 = as.a0;
   op.v1 = as.a1;
   op.v2 = as.a2;
-  special_point<CF, WF>(df, spf, sc, corr, pred, step, energy, i_f, bm, ep, ek, op);
+  special_finish<CF, WF>(df, spf, sc, corr, step, energy, i_f, b, ep, ek, op);
 }
 
 // Coupled model, fluid special points off the interface (the interface ones
