@@ -96,8 +96,14 @@ BlockCache g_cache[kCacheDevices];
 BlockCache* cache_of(int device) { return device >= 0 && device < kCacheDevices ? &g_cache[device] : nullptr; }
 
 // Blocks reserved for staged contexts (insem_hip_reserve): address ->
-// (device, size).  Guarded by g_capture_mu.
-std::map<void*, std::pair<int, size_t>> g_reserved;
+// device, size and the staged context laid out in it (null until
+// insem_hip_stage claims it).  Guarded by g_capture_mu.
+struct Reservation {
+  int device = 0;
+  size_t size = 0;
+  const void* owner = nullptr;
+};
+std::map<void*, Reservation> g_reserved;
 
 // Allocation granularity: 4 KiB up to 1 MiB, 2 MiB above.
 size_t block_bytes(size_t bytes) {
@@ -1407,7 +1413,7 @@ int insem_hip_reserve(int64_t bytes, void** block) {
     bool reused = false;
     void* p = take_block(dev, static_cast<size_t>(bytes), &size, &reused);
     std::lock_guard<std::mutex> lock(g_capture_mu);
-    g_reserved[p] = {dev, size};
+    g_reserved[p] = Reservation{dev, size, nullptr};
     *block = p;
     return INSEM_HIP_OK;
   } catch (const HipFailure& e) {
@@ -1416,15 +1422,17 @@ int insem_hip_reserve(int64_t bytes, void** block) {
 }
 
 int insem_hip_unreserve(void* block) {
-  std::pair<int, size_t> r;
+  Reservation r;
   {
     std::lock_guard<std::mutex> lock(g_capture_mu);
     const auto it = g_reserved.find(block);
     if (it == g_reserved.end()) return set_error(INSEM_HIP_EINVAL, "the block is not reserved");
+    if (it->second.owner)
+      return set_error(INSEM_HIP_ESTATE, "the block holds a staged context: insem_hip_destroy it instead");
     r = it->second;
     g_reserved.erase(it);
   }
-  give_block(r.first, block, r.second);
+  give_block(r.device, block, r.size);
   return INSEM_HIP_OK;
 }
 
@@ -1450,7 +1458,7 @@ int insem_hip_realize(insem_hip_ctx* c) {
   }
   {
     std::lock_guard<std::mutex> lock(g_capture_mu);
-    g_reserved.erase(block);
+    g_reserved.erase(block);  // the context owns the block now
   }
   c->allocs.emplace_back(block, c->stage_cap);
   c->staging = 0;
@@ -1554,6 +1562,15 @@ struct BatchOrderedSetup {
   }
 };
 
+// A staged context's claim on its reserved block, given up (the block stays
+// reserved for its caller).
+void release_claim(const insem_hip_ctx* c) {
+  if (c->staging != 2) return;
+  std::lock_guard<std::mutex> lock(g_capture_mu);
+  const auto it = g_reserved.find(reinterpret_cast<void*>(c->stage_base));
+  if (it != g_reserved.end() && it->second.owner == c) it->second.owner = nullptr;
+}
+
 // A context of the setup (and batch): on the current device (staging 0),
 // or without any device call, measured (1) or laid out for the reserved
 // block `block` (2) for insem_hip_realize.
@@ -1570,12 +1587,13 @@ int create_impl(const insem_hip_setup* s0, const insem_hip_batch* batch, insem_h
   if (staging == 2) {
     std::lock_guard<std::mutex> lock(g_capture_mu);
     const auto it = g_reserved.find(block);
-    if (it == g_reserved.end()) {
+    if (it == g_reserved.end() || it->second.owner) {
       delete c;
       return set_error(INSEM_HIP_EINVAL, "the block was not reserved with insem_hip_reserve (or is in use)");
     }
-    c->device = it->second.first;
-    c->stage_cap = it->second.second;
+    it->second.owner = c;
+    c->device = it->second.device;
+    c->stage_cap = it->second.size;
     c->stage_base = reinterpret_cast<uintptr_t>(block);
   }
   try {
@@ -1630,10 +1648,12 @@ int create_impl(const insem_hip_setup* s0, const insem_hip_batch* batch, insem_h
     if (const char* f = std::getenv("INSEM_HIP_PAIR")) c->pair = std::atoi(f) != 0;
     if (!staging) HIP_TRY(hipStreamSynchronize(c->stream));
   } catch (const HipFailure& e) {
+    release_claim(c);
     free_all(c);
     delete c;
     return e.code;
   } catch (const std::exception& e) {
+    release_claim(c);
     free_all(c);
     delete c;
     return set_error(INSEM_HIP_EDEVICE, "%s", e.what());
@@ -1689,7 +1709,10 @@ int insem_hip_destroy(insem_hip_ctx* c) {
   if (!c) return INSEM_HIP_OK;
   if (c->staging) {
     // never realized: no device resources but its reserved block
-    if (c->staging == 2) (void)insem_hip_unreserve(reinterpret_cast<void*>(c->stage_base));
+    if (c->staging == 2) {
+      release_claim(c);
+      (void)insem_hip_unreserve(reinterpret_cast<void*>(c->stage_base));
+    }
     delete c;
     return INSEM_HIP_OK;
   }

@@ -59,6 +59,12 @@ def test_staged_context_refuses_calls_and_gives_its_block_back():
     _lib.check_hip(hip.insem_hip_stage(ctypes.c_void_p(img.view), ctypes.c_void_p(block), ctypes.byref(ctx)))
     assert hip.insem_hip_advance(ctx, 1) != 0
     assert b"staged" in hip.insem_hip_last_error()
+    # one staged context per block; a claimed block is not given back by hand
+    ctx2 = ctypes.c_void_p()
+    assert hip.insem_hip_stage(ctypes.c_void_p(img.view), ctypes.c_void_p(block), ctypes.byref(ctx2)) != 0
+    assert b"in use" in hip.insem_hip_last_error()
+    with pytest.raises(insem.InsemError, match="staged context"):
+        unreserve_block(block)
     _lib.check_hip(hip.insem_hip_destroy(ctx))  # never realized: the block goes back
     with pytest.raises(insem.InsemError, match="not reserved"):
         unreserve_block(block)
